@@ -1,0 +1,276 @@
+"""Benchmark of the MI355X block-DDA solve path.
+
+A step is one time-to-solution: the block COCG-rq solve of the whole
+right-hand-side block to a true FP64 relative residual ||B - A X||/||B|| of
+1e-5, with B already resident in HBM and the transformed interaction kernel
+cached (the reference also builds it before its solve timer,
+Solver.cpp:55,59).  The default workload is the 16384-monomer aggregate
+(a0 = 20 nm, Df = 2.8, kf = 1.0, seeded tunable aggregation, 4 dipoles across
+each monomer, m = 2.3 + 0.6i, lambda = 400 nm, LDR polarizability, s = 16:
+8 seeded directions x 2 polarisations), solved in mixed precision (complex
+FP32 inner block iteration inside an FP64 defect-correction loop); the same
+solve in pure FP64 is timed beside it.
+
+Printed JSON line: metric/value (mixed time-to-solution, seconds), the FP64
+time-to-solution and the speedup, a roofline object for the dominant kernel
+(the fused x pass of the FP32 block matvec, HIP events on the library
+stream), and a cpu_baseline measured on this host with the oracle (the FP64
+C restatement of the reference algorithm: one 1-D FFT of length
+f^2 prod(2n-1) per RHS per matvec, OpenMP over RHS columns).
+"""
+import argparse
+import ctypes
+import importlib
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "time-to-solution & p99 latency at rel.res 1e-5; mixed-prec vs FP64 on MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    name = "block_dda_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    d = os.path.join(ROOT, "block-dda_amd")
+    spec = importlib.util.spec_from_file_location(name, os.path.join(d, "__init__.py"),
+                                                  submodule_search_locations=[d])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+CONFIGS = {
+    # name: (monomers, dipoles_across, m, wavelength_nm, directions, seed)
+    "agg16384": dict(N=16384, Df=2.8, kf=1.0, a0=20.0, across=4, m=2.3 + 0.6j, wl=400.0, ndir=8, seed=16384),
+    "agg1024": dict(N=1024, Df=2.8, kf=1.0, a0=20.0, across=8, m=1.95 + 0.79j, wl=550.0, ndir=16, seed=1024),
+    "agg256": dict(N=256, Df=2.8, kf=1.0, a0=20.0, across=4, m=2.3 + 0.6j, wl=400.0, ndir=8, seed=256),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(pkg, cfg):
+    T = importlib.import_module("block_dda_amd.targets")
+    t0 = time.time()
+    tgt = T.aggregate_target(cfg["N"], cfg["Df"], cfg["kf"], cfg["a0"], cfg["across"], cfg["seed"])
+    k = 2 * math.pi / cfg["wl"]
+    alpha = T.alpha_ldr(cfg["m"], tgt.lf, k, S=0.0)
+    diag = T.diag_from_alpha(alpha, tgt.nocc, 3)
+    waves = T.plane_waves_sphere(k, cfg["ndir"], cfg["seed"])
+    B = waves.rhs(tgt)
+    log("workload: %s grid %s nocc %d s %d (%.1fs)" % (tgt.name, tgt.n, tgt.nocc, B.shape[1], time.time() - t0))
+    return tgt, k, diag, B
+
+
+def pass_bytes(n, P, nocc, s, f, c, nc):
+    nx, ny, nz = n
+    Px, Py, Pz = P
+    nxnyPz = nx * ny * Pz
+    nxPyPz = nx * Py * Pz
+    gh = nc * (Px // 2 + 1) * (Py // 2 + 1) * (Pz // 2 + 1) * c
+    return {
+        "z_fwd": s * f * c * (nocc + nxnyPz),
+        "y_fwd": s * f * c * (nxnyPz + nxPyPz),
+        "x_fused": s * f * c * 2 * nxPyPz + gh,
+        "y_inv": s * f * c * (nxPyPz + nxnyPz),
+        "z_inv": s * f * c * (nxnyPz + 2 * nocc),
+    }
+
+
+def time_matvec(pkg, op, rows, s, prec, reps):
+    L = pkg.lib()
+    c = 8 if prec == pkg.BDDA_PREC_FP32 else 16
+    dX = ctypes.c_void_p()
+    dY = ctypes.c_void_p()
+    nbytes = rows * s * c
+    pkg._check(L.bdda_dev_alloc(nbytes, ctypes.byref(dX)))
+    pkg._check(L.bdda_dev_alloc(nbytes, ctypes.byref(dY)))
+    rng = np.random.default_rng(0)
+    if prec == pkg.BDDA_PREC_FP32:
+        h = (rng.standard_normal((rows * s, 2)).astype(np.float32)).ravel()
+    else:
+        h = rng.standard_normal(rows * s * 2)
+    pkg._check(L.bdda_memcpy_h2d(dX, h.ctypes.data, nbytes))
+    pkg._check(L.bdda_mbt_set_timing(op.h, 1))
+    names = ["z_fwd", "y_fwd", "x_fused", "y_inv", "z_inv"]
+    acc = np.zeros(5)
+    pt = (ctypes.c_float * 5)()
+    pkg._check(L.bdda_mbt_apply_occupied_dev(op.h, prec, 1, dX, dY, s))  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pkg._check(L.bdda_mbt_apply_occupied_dev(op.h, prec, 1, dX, dY, s))
+        pkg._check(L.bdda_mbt_pass_times(op.h, pt))
+        acc += np.array(pt[:])
+    wall = (time.perf_counter() - t0) / reps
+    pkg._check(L.bdda_mbt_set_timing(op.h, 0))
+    L.bdda_dev_free(dX)
+    L.bdda_dev_free(dY)
+    ms = acc / reps
+    return dict(zip(names, ms.tolist())), wall * 1e3
+
+
+def solve_dev(pkg, op, dB, dX, s, tol, itermax, mode, method=0, jacobi=1):
+    L = pkg.lib()
+    info = pkg.SolveInfo()
+    pkg._check(L.bdda_device_sync())
+    t0 = time.perf_counter()
+    rc = L.bdda_solve_dev(op.h, method, jacobi, mode, dB, s, tol, itermax, dX, ctypes.byref(info), None, 0)
+    pkg._check(L.bdda_device_sync())
+    wall = time.perf_counter() - t0
+    if rc not in (0, 5):
+        pkg._check(rc)
+    d = info.as_dict()
+    d["wall_s"] = wall
+    return d
+
+
+def cpu_baseline(tgt, k, diag, s, n_matvecs, threads):
+    """Oracle (FP64 C restatement of the reference) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    T = importlib.import_module("block_dda_amd.targets")
+    # sample: the same generator and lattice spacing at 512 monomers, one
+    # block matvec of `threads` RHS columns (OpenMP over columns, like the
+    # reference); scaled to the full target by F log F of the 1-D FFT.
+    cfg = dict(N=512, Df=2.8, kf=1.0, a0=20.0, across=4, seed=512)
+    st = T.aggregate_target(cfg["N"], cfg["Df"], cfg["kf"], cfg["a0"], cfg["across"], cfg["seed"])
+    sdiag = T.diag_from_alpha(1.0 / diag[0], st.nocc, 3)
+    Au = orc.mbt_fft_init(st.n, 3, st.lf, k)
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((3 * st.nocc, threads)) + 1j * rng.standard_normal((3 * st.nocc, threads))
+    t0 = time.perf_counter()
+    orc.block_matvec(st.n, 3, st.address, Au, sdiag, X, jacobi=True, nthreads=threads)
+    t_block = time.perf_counter() - t0
+    Fs = orc.fft_length(st.n, 3)
+    Ff = orc.fft_length(tgt.n, 3)
+    scale = (Ff * math.log2(Ff)) / (Fs * math.log2(Fs))
+    t_full = t_block * scale * math.ceil(s / threads) * n_matvecs
+    return {
+        "value": t_full, "unit": "s", "cores": threads, "kind": "port",
+        "sample": ("oracle block matvec (FP64, 1-D FFT of length F=%d, %d RHS on %d OpenMP threads) on the "
+                   "512-monomer aggregate at the same lattice spacing: %.2f s; scaled by F log F to F=%d and by "
+                   "%d RHS x %d block matvecs of the FP64 GPU solve (extrapolated, solve only)"
+                   % (Fs, threads, threads, t_block, Ff, s, n_matvecs)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="agg16384", choices=sorted(CONFIGS))
+    ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--itermax", type=int, default=3000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--matvec-reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the solve path is measured on a single MI355X "
+              "(multi-GPU sharding by RHS columns is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+    pkg = load_pkg()
+    cfg = CONFIGS[args.config]
+    tgt, k, diag, B = build_workload(pkg, cfg)
+    s = B.shape[1]
+    t0 = time.time()
+    op = pkg.MBT_fft_init(tgt.n, 3, tgt.lf, k)
+    op.set_target(tgt.address, diag)
+    pkg._check(pkg.lib().bdda_device_sync())
+    init_s = time.time() - t0
+    P = op.grid
+    log("operator: padded grid %s, init %.2fs" % (P, init_s))
+    rows = 3 * tgt.nocc
+    L = pkg.lib()
+    dB = ctypes.c_void_p()
+    dX = ctypes.c_void_p()
+    pkg._check(L.bdda_dev_alloc(rows * s * 16, ctypes.byref(dB)))
+    pkg._check(L.bdda_dev_alloc(rows * s * 16, ctypes.byref(dX)))
+    Bh = np.asfortranarray(B)
+    pkg._check(L.bdda_memcpy_h2d(dB, Bh.ctypes.data, rows * s * 16))
+
+    # per-pass kernel times of the block matvec (HIP events, library stream)
+    mv = {}
+    for name, prec, c in (("fp32", pkg.BDDA_PREC_FP32, 8), ("fp64", pkg.BDDA_PREC_FP64, 16)):
+        ms, wall = time_matvec(pkg, op, rows, s, prec, args.matvec_reps)
+        by = pass_bytes(tgt.n, P, tgt.nocc, s, 3, c, 6)
+        mv[name] = {"pass_ms": ms, "wall_ms": wall,
+                    "pass_GBs": {kk: by[kk] / (ms[kk] * 1e-3) / 1e9 for kk in ms},
+                    "matvec_GBs": sum(by.values()) / (sum(ms.values()) * 1e-3) / 1e9,
+                    "bytes": by}
+        log("matvec %s: %s  total %.2f ms (%.0f GB/s algorithmic)" % (
+            name, {kk: round(v, 3) for kk, v in ms.items()}, sum(ms.values()), mv[name]["matvec_GBs"]))
+
+    # FP64 reference-semantics solve (recurrence residual < tol)
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_FP64)
+        log("fp64 solve: %s" % fp64)
+    # mixed precision (true FP64 residual <= tol): warmup + timed steps
+    for _ in range(args.warmup):
+        r = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_MIXED)
+        log("mixed warmup: %s" % r)
+    runs = []
+    pkg._check(L.bdda_device_sync())
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        runs.append(solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_MIXED))
+        log("mixed step: %s" % runs[-1])
+    pkg._check(L.bdda_device_sync())
+    total = time.perf_counter() - t_all
+    ms_per_step = total / max(args.steps, 1) * 1e3
+    tts = ms_per_step / 1e3
+
+    xf = mv["fp32"]
+    dom_bytes = xf["bytes"]["x_fused"]
+    dom_ms = xf["pass_ms"]["x_fused"]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_xfused<float,3> (x FFT fwd + G^ multiply + x FFT inv)",
+                "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms}
+    cpu = None
+    if not args.no_cpu:
+        try:
+            nmv = fp64["matvecs"] if fp64 else runs[-1]["matvecs"]
+            cpu = cpu_baseline(tgt, k, diag, s, nmv, threads=min(s, 16, os.cpu_count() or 1))
+        except Exception as e:  # the baseline is reported, never required
+            log("cpu baseline failed: %r" % e)
+    out = {
+        "metric": METRIC, "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex64 inner + complex128 outer", "data": "synthetic (seeded aggregate, plane waves)",
+        "config": {"workload": "%s: %d monomers a0=20nm Df=2.8 kf=1.0, %d dipoles across, grid %s -> %s, "
+                               "N=%d dipoles, m=%s, lambda=%gnm, s=%d, block COCG-rq (Jacobi), tol %g"
+                               % (args.config, cfg["N"], cfg["across"], list(tgt.n), list(P), tgt.nocc,
+                                  cfg["m"], cfg["wl"], s, args.tol),
+                   "parallelism": "single GPU"},
+        "mixed": {"iters": runs[-1]["iters"], "outer": runs[-1]["outer_iters"], "matvecs": runs[-1]["matvecs"],
+                  "true_res": runs[-1]["true_res"], "converged": runs[-1]["converged"]},
+        "fp64_time_s": fp64["wall_s"] if fp64 else None,
+        "fp64": {k2: fp64[k2] for k2 in ("iters", "matvecs", "true_res", "recurrence_res", "converged")}
+        if fp64 else None,
+        "speedup_mixed_vs_fp64": (fp64["wall_s"] / tts) if fp64 else None,
+        "init_s": init_s,
+        "matvec": {kk: {"pass_ms": v["pass_ms"], "matvec_GBs": v["matvec_GBs"],
+                        "pass_GBs": v["pass_GBs"]} for kk, v in mv.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

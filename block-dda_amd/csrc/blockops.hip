@@ -1,0 +1,624 @@
+// blockops.hip -- block-vector kernels (Gram, linear combinations,
+// reductions, reduced QR) for the block Krylov solvers on gfx950.
+//
+// Reference operations replaced (paths relative to the reference root):
+//   S.transpose()*AS, Q.transpose()*Q      bl_cocg_rq_*_mvp_fft.cpp:56,63,66,73
+//   R0til_H*V, R0til_H*R, R0til_H*Z        bl_bicgstab_*:58-59,93 / bl_bicggr_*:51,110
+//   (Z.adjoint()*T).trace()                bl_bicgstab_*:87, bl_bicggr_*:52
+//   X + S*alpha*xi, Q - AS*alpha, ...      the block updates of every solver
+//   B.norm()                               Frobenius norms
+//   qr_reduced                             bl_krylov_mvp_fft/qr_reduced.cpp:9-31
+#include "blockops.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace bdda {
+
+constexpr int NT_RED = 256;
+constexpr int MAX_PARTIAL_BLOCKS = 1024;
+constexpr int COEF_SLOTS = 64;
+constexpr int COEF_SLOT_ELEMS = 2 * 64 * 64;
+
+BlockWork::~BlockWork()
+{
+    if (partial) (void)hipFree(partial);
+    if (red) (void)hipFree(red);
+    if (coef) (void)hipFree(coef);
+    if (host_pinned) (void)hipHostFree(host_pinned);
+}
+
+void BlockWork::reserve(size_t partial_n, size_t red_n)
+{
+    if (partial_n > partial_elems) {
+        if (partial) BDDA_HIP(hipFree(partial));
+        partial = nullptr;
+        BDDA_HIP(hipMalloc(&partial, sizeof(c64) * partial_n));
+        partial_elems = partial_n;
+    }
+    if (red_n > red_elems) {
+        if (red) BDDA_HIP(hipFree(red));
+        red = nullptr;
+        BDDA_HIP(hipMalloc(&red, sizeof(c64) * red_n));
+        red_elems = red_n;
+    }
+    if (!coef) {
+        BDDA_HIP(hipMalloc(&coef, sizeof(c64) * COEF_SLOTS * COEF_SLOT_ELEMS));
+        BDDA_HIP(hipHostMalloc(&host_pinned, sizeof(c64) * (COEF_SLOTS * COEF_SLOT_ELEMS + 64 * 64),
+                               hipHostMallocDefault));
+        host_elems = COEF_SLOTS * COEF_SLOT_ELEMS;
+    }
+}
+
+// ------------------------------------------------------------- kernels
+template <typename R, bool CONJ>
+__device__ __forceinline__ c64 cmul64(cplx<R> a, cplx<R> b)
+{
+    const double ax = a.x, ay = CONJ ? -(double)a.y : (double)a.y, bx = b.x, by = b.y;
+    return mk(ax * bx - ay * by, ax * by + ay * bx);
+}
+
+template <typename R, bool CONJ>
+__global__ __launch_bounds__(NT_RED) void k_gram_partial(const cplx<R>* __restrict__ A, int64_t lda, int a,
+                                                        const cplx<R>* __restrict__ B, int64_t ldb, int b,
+                                                        int64_t rows, int TR, int64_t rows_per_blk,
+                                                        c64* __restrict__ partial)
+{
+    constexpr int PPT = 16;  // pairs per thread (a*b <= 4096)
+    extern __shared__ unsigned char smem_raw[];
+    cplx<R>* As = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* Bs = As + (size_t)TR * a;
+    const int np = a * b;
+    c64 acc[PPT];
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) acc[i] = mk(0.0, 0.0);
+    const int64_t rb = blockIdx.x * rows_per_blk;
+    const int64_t re = min(rows, rb + rows_per_blk);
+    for (int64_t r0 = rb; r0 < re; r0 += TR) {
+        const int nr = (int)min((int64_t)TR, re - r0);
+        for (int e = threadIdx.x; e < TR * a; e += NT_RED) {
+            const int r = e % TR, col = e / TR;
+            As[r * a + col] = (r < nr) ? A[col * lda + r0 + r] : czero<R>();
+        }
+        for (int e = threadIdx.x; e < TR * b; e += NT_RED) {
+            const int r = e % TR, col = e / TR;
+            Bs[r * b + col] = (r < nr) ? B[col * ldb + r0 + r] : czero<R>();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const int p = threadIdx.x + i * NT_RED;
+            if (p < np) {
+                const int ia = p % a, ib = p / a;
+                c64 s = acc[i];
+                for (int r = 0; r < nr; ++r) {
+                    const c64 t = cmul64<R, CONJ>(As[r * a + ia], Bs[r * b + ib]);
+                    s.x += t.x;
+                    s.y += t.y;
+                }
+                acc[i] = s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        const int p = threadIdx.x + i * NT_RED;
+        if (p < np) partial[(int64_t)blockIdx.x * np + p] = acc[i];
+    }
+}
+
+__global__ void k_reduce_partial(const c64* __restrict__ partial, int nblk, int np, c64* __restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    double sx = 0, sy = 0;
+    for (int k = 0; k < nblk; ++k) {
+        const c64 v = partial[(int64_t)k * np + p];
+        sx += v.x;
+        sy += v.y;
+    }
+    out[p] = mk(sx, sy);
+}
+
+template <typename R, bool CONJ>
+__global__ __launch_bounds__(NT_RED) void k_dot_partial(const cplx<R>* __restrict__ A, int64_t lda,
+                                                       const cplx<R>* __restrict__ B, int64_t ldb, int64_t rows,
+                                                       int s, c64* __restrict__ partial)
+{
+    __shared__ double sh[2][NT_RED];
+    double sx = 0, sy = 0;
+    const int64_t total = rows * s;
+    for (int64_t i = blockIdx.x * (int64_t)NT_RED + threadIdx.x; i < total; i += (int64_t)gridDim.x * NT_RED) {
+        const int64_t r = i % rows, j = i / rows;
+        const c64 t = cmul64<R, CONJ>(A[j * lda + r], B[j * ldb + r]);
+        sx += t.x;
+        sy += t.y;
+    }
+    sh[0][threadIdx.x] = sx;
+    sh[1][threadIdx.x] = sy;
+    __syncthreads();
+    for (int w = NT_RED / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = mk(sh[0][0], sh[1][0]);
+}
+
+template <typename R>
+struct ComboDev {
+    int nv;
+    const cplx<R>* V[3];
+    int64_t ldv[3];
+    cplx<R> a[3];
+    int nm;
+    const cplx<R>* W[2];
+    int64_t ldw[2];
+    int kw[2];
+    const cplx<R>* M[2];  // device, kw x s column-major, already in precision R
+};
+
+// Z[:, j0 + jj] = sum_t a_t V_t[:, j0+jj] + sum_u sum_l W_u[:, l] M_u[l, j0+jj]
+// for jj < nj <= SJ.  One row per thread; the coefficient reads are
+// wave-uniform.  Z may alias a V_t (same element, same thread) but not a W_u.
+template <typename R, int SJ>
+__global__ __launch_bounds__(256) void k_combo(ComboDev<R> c, cplx<R>* Z, int64_t ldz, int64_t rows, int j0, int nj)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+        cplx<R> acc[SJ];
+#pragma unroll
+        for (int jj = 0; jj < SJ; ++jj) {
+            acc[jj] = czero<R>();
+            if (jj < nj)
+                for (int t = 0; t < c.nv; ++t) fma_c(acc[jj], c.a[t], c.V[t][(int64_t)(j0 + jj) * c.ldv[t] + r]);
+        }
+        for (int u = 0; u < c.nm; ++u) {
+            const int kw = c.kw[u];
+            const cplx<R>* W = c.W[u];
+            const cplx<R>* M = c.M[u];
+            for (int l = 0; l < kw; ++l) {
+                const cplx<R> w = W[(int64_t)l * c.ldw[u] + r];
+#pragma unroll
+                for (int jj = 0; jj < SJ; ++jj)
+                    if (jj < nj) fma_c(acc[jj], w, M[l + (int64_t)(j0 + jj) * kw]);
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < SJ; ++jj)
+            if (jj < nj) Z[(int64_t)(j0 + jj) * ldz + r] = acc[jj];
+    }
+}
+
+template <typename R>
+__global__ void k_scale_rows(const cplx<R>* __restrict__ d, const cplx<R>* A, int64_t lda, cplx<R>* Z, int64_t ldz,
+                             int64_t rows, int s)
+{
+    const int64_t total = rows * s;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i % rows, j = i / rows;
+        Z[j * ldz + r] = A[j * lda + r] * d[r];
+    }
+}
+
+__global__ void k_cvt_64_32(const c64* __restrict__ A, int64_t lda, c32* __restrict__ Z, int64_t ldz, int64_t rows,
+                            int s, double scale)
+{
+    const int64_t total = rows * s;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i % rows, j = i / rows;
+        const c64 v = A[j * lda + r];
+        Z[j * ldz + r] = mk((float)(v.x * scale), (float)(v.y * scale));
+    }
+}
+
+__global__ void k_acc_32_64(const c32* __restrict__ A, int64_t lda, c64* __restrict__ Z, int64_t ldz, int64_t rows,
+                            int s, double scale)
+{
+    const int64_t total = rows * s;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i % rows, j = i / rows;
+        const c32 v = A[j * lda + r];
+        c64 z = Z[j * ldz + r];
+        z.x += scale * (double)v.x;
+        z.y += scale * (double)v.y;
+        Z[j * ldz + r] = z;
+    }
+}
+
+template <typename R>
+__global__ void k_zero(cplx<R>* Z, int64_t ldz, int64_t rows, int s)
+{
+    const int64_t total = rows * s;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i % rows, j = i / rows;
+        Z[j * ldz + r] = czero<R>();
+    }
+}
+
+// ------------------------------------------------------------- host
+static int grid_elem(int64_t n)
+{
+    return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 8192);
+}
+
+template <typename R>
+HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* B, int64_t ldb, int b,
+             int64_t rows, int conjA)
+{
+    const int np = a * b;
+    if (a < 1 || b < 1 || np > 4096) throw HipError{hipErrorInvalidValue, "gram: block too wide"};
+    const size_t c = sizeof(cplx<R>);
+    int TR = (int)std::min<int64_t>(256, 32768 / ((a + b) * c));
+    TR = std::max(TR, 1);
+    int64_t rpb = std::max<int64_t>(TR, ceil_div(std::max<int64_t>(rows, 1), MAX_PARTIAL_BLOCKS));
+    rpb = ceil_div(rpb, TR) * TR;
+    const int nblk = (int)std::max<int64_t>(1, ceil_div(rows, rpb));
+    w.reserve((size_t)nblk * np, np);
+    const size_t shm = c * (size_t)TR * (a + b);
+    if (conjA)
+        hipLaunchKernelGGL((k_gram_partial<R, true>), dim3(nblk), dim3(NT_RED), shm, w.stream, A, lda, a, B, ldb, b,
+                           rows, TR, rpb, w.partial);
+    else
+        hipLaunchKernelGGL((k_gram_partial<R, false>), dim3(nblk), dim3(NT_RED), shm, w.stream, A, lda, a, B, ldb, b,
+                           rows, TR, rpb, w.partial);
+    BDDA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0, w.stream,
+                       (const c64*)w.partial, nblk, np, w.red);
+    BDDA_HIP(hipGetLastError());
+    HostMat out(np);
+    BDDA_HIP(hipMemcpyAsync(w.host_pinned + w.host_elems, w.red, sizeof(c64) * np, hipMemcpyDeviceToHost, w.stream));
+    BDDA_HIP(hipStreamSynchronize(w.stream));
+    std::memcpy(out.data(), w.host_pinned + w.host_elems, sizeof(c64) * np);
+    return out;
+}
+
+template <typename R>
+c64 dot_all(BlockWork& w, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb, int64_t rows, int s,
+            int conjA)
+{
+    const int nblk = (int)std::min<int64_t>(MAX_PARTIAL_BLOCKS, std::max<int64_t>(1, ceil_div(rows * s, NT_RED)));
+    w.reserve(nblk, 1);
+    if (conjA)
+        hipLaunchKernelGGL((k_dot_partial<R, true>), dim3(nblk), dim3(NT_RED), 0, w.stream, A, lda, B, ldb, rows, s,
+                           w.partial);
+    else
+        hipLaunchKernelGGL((k_dot_partial<R, false>), dim3(nblk), dim3(NT_RED), 0, w.stream, A, lda, B, ldb, rows, s,
+                           w.partial);
+    BDDA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partial, dim3(1), dim3(256), 0, w.stream, (const c64*)w.partial, nblk, 1, w.red);
+    BDDA_HIP(hipGetLastError());
+    BDDA_HIP(hipMemcpyAsync(w.host_pinned + w.host_elems, w.red, sizeof(c64), hipMemcpyDeviceToHost, w.stream));
+    BDDA_HIP(hipStreamSynchronize(w.stream));
+    return w.host_pinned[w.host_elems];
+}
+
+template <typename R>
+double norm_fro(BlockWork& w, const cplx<R>* A, int64_t lda, int64_t rows, int s)
+{
+    const c64 v = dot_all<R>(w, A, lda, A, lda, rows, s, 1);
+    return std::sqrt(std::max(v.x, 0.0));
+}
+
+
+template <typename R>
+void combo(BlockWork& w, const Combo<R>& c, cplx<R>* Z, int64_t ldz, int64_t rows, int s)
+{
+    w.reserve(1, 1);
+    ComboDev<R> cd{};
+    cd.nv = c.nv;
+    for (int t = 0; t < c.nv; ++t) {
+        cd.V[t] = c.V[t];
+        cd.ldv[t] = c.ldv[t];
+        cd.a[t] = mk((R)c.a[t].x, (R)c.a[t].y);
+    }
+    cd.nm = c.nm;
+    if (c.nm > 0) {
+        // stage the coefficient matrices (converted to R) through a rotating
+        // pinned slot; wrap-around waits for the stream so a slot is never
+        // rewritten while its copy may still be pending
+        if (w.coef_slot == 0) BDDA_HIP(hipStreamSynchronize(w.stream));
+        cplx<R>* hslot = reinterpret_cast<cplx<R>*>(w.host_pinned + (size_t)w.coef_slot * COEF_SLOT_ELEMS);
+        cplx<R>* dslot = reinterpret_cast<cplx<R>*>(w.coef + (size_t)w.coef_slot * COEF_SLOT_ELEMS);
+        w.coef_slot = (w.coef_slot + 1) % COEF_SLOTS;
+        size_t off = 0;
+        for (int u = 0; u < c.nm; ++u) {
+            cd.W[u] = c.W[u];
+            cd.ldw[u] = c.ldw[u];
+            cd.kw[u] = c.kw[u];
+            const size_t n = (size_t)c.kw[u] * s;
+            if (c.kw[u] > 64 || s > 64) throw HipError{hipErrorInvalidValue, "combo: block wider than 64"};
+            for (size_t e = 0; e < n; ++e) hslot[off + e] = mk((R)c.M[u][e].x, (R)c.M[u][e].y);
+            cd.M[u] = dslot + off;
+            off += n;
+        }
+        BDDA_HIP(hipMemcpyAsync(dslot, hslot, sizeof(cplx<R>) * off, hipMemcpyHostToDevice, w.stream));
+    }
+    constexpr int SJ = 16;
+    for (int j0 = 0; j0 < s; j0 += SJ) {
+        const int nj = std::min(SJ, s - j0);
+        if (nj <= 8 && j0 == 0 && s <= 8)
+            hipLaunchKernelGGL((k_combo<R, 8>), dim3(grid_elem(rows)), dim3(256), 0, w.stream, cd, Z, ldz, rows, j0,
+                               nj);
+        else
+            hipLaunchKernelGGL((k_combo<R, SJ>), dim3(grid_elem(rows)), dim3(256), 0, w.stream, cd, Z, ldz, rows, j0,
+                               nj);
+        BDDA_HIP(hipGetLastError());
+    }
+}
+
+template <typename R>
+void scale_rows(hipStream_t st, const cplx<R>* d, const cplx<R>* A, int64_t lda, cplx<R>* Z, int64_t ldz,
+                int64_t rows, int s)
+{
+    hipLaunchKernelGGL(k_scale_rows<R>, dim3(grid_elem(rows * s)), dim3(256), 0, st, d, A, lda, Z, ldz, rows, s);
+    BDDA_HIP(hipGetLastError());
+}
+
+void convert_64_to_32(hipStream_t st, const c64* A, int64_t lda, c32* Z, int64_t ldz, int64_t rows, int s,
+                      double scale)
+{
+    hipLaunchKernelGGL(k_cvt_64_32, dim3(grid_elem(rows * s)), dim3(256), 0, st, A, lda, Z, ldz, rows, s, scale);
+    BDDA_HIP(hipGetLastError());
+}
+
+void accumulate_32_to_64(hipStream_t st, const c32* A, int64_t lda, c64* Z, int64_t ldz, int64_t rows, int s,
+                         double scale)
+{
+    hipLaunchKernelGGL(k_acc_32_64, dim3(grid_elem(rows * s)), dim3(256), 0, st, A, lda, Z, ldz, rows, s, scale);
+    BDDA_HIP(hipGetLastError());
+}
+
+template <typename R>
+void fill_zero(hipStream_t st, cplx<R>* Z, int64_t ldz, int64_t rows, int s)
+{
+    hipLaunchKernelGGL(k_zero<R>, dim3(grid_elem(rows * s)), dim3(256), 0, st, Z, ldz, rows, s);
+    BDDA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- QR
+template <typename R>
+int qr_cgs2(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s, HostMat& Rm)
+{
+    // a column whose norm after re-orthogonalisation falls below this
+    // fraction of its original norm is numerically dependent: breakdown
+    const double dep = (sizeof(R) == 8) ? 1e-12 : 1e-5;
+    Rm.assign((size_t)s * s, mk(0.0, 0.0));
+    for (int k = 0; k < s; ++k) {
+        cplx<R>* qk = Q + k * ldq;
+        const double orig = norm_fro<R>(w, A + k * lda, lda, rows, 1);
+        if ((const void*)A != (const void*)Q || lda != ldq) {
+            Combo<R> cp;
+            cp.nv = 1;
+            cp.V[0] = A + k * lda;
+            cp.ldv[0] = lda;
+            cp.a[0] = mk(1.0, 0.0);
+            combo<R>(w, cp, qk, ldq, rows, 1);
+        }
+        for (int pass = 0; pass < 2 && k > 0; ++pass) {
+            HostMat dk = gram<R>(w, Q, ldq, k, qk, ldq, 1, rows, 1);
+            if (pass == 0)
+                for (int i = 0; i < k; ++i) Rm[i + (size_t)k * s] = dk[i];
+            for (auto& v : dk) v = mk(-v.x, -v.y);
+            Combo<R> up;
+            up.nv = 1;
+            up.V[0] = qk;
+            up.ldv[0] = ldq;
+            up.a[0] = mk(1.0, 0.0);
+            up.nm = 1;
+            up.W[0] = Q;
+            up.ldw[0] = ldq;
+            up.kw[0] = k;
+            up.M[0] = dk.data();
+            combo<R>(w, up, qk, ldq, rows, 1);
+        }
+        const double nrm = norm_fro<R>(w, qk, ldq, rows, 1);
+        if (!(nrm > dep * orig) || !std::isfinite(nrm)) return BDDA_EBREAKDOWN;
+        Rm[k + (size_t)k * s] = mk(nrm, 0.0);
+        Combo<R> sc;
+        sc.nv = 1;
+        sc.V[0] = qk;
+        sc.ldv[0] = ldq;
+        sc.a[0] = mk(1.0 / nrm, 0.0);
+        combo<R>(w, sc, qk, ldq, rows, 1);
+    }
+    return BDDA_OK;
+}
+
+// upper Cholesky of a Hermitian positive definite G = U^H U; false on failure
+static bool chol_upper(const HostMat& G, int s, HostMat& U, double rel_floor)
+{
+    U.assign((size_t)s * s, mk(0.0, 0.0));
+    for (int j = 0; j < s; ++j) {
+        double d = G[j + (size_t)j * s].x;
+        const double gjj = d;
+        for (int k = 0; k < j; ++k) {
+            const c64 u = U[k + (size_t)j * s];
+            d -= u.x * u.x + u.y * u.y;
+        }
+        if (!(d > rel_floor * gjj) || !std::isfinite(d)) return false;
+        const double ujj = std::sqrt(d);
+        U[j + (size_t)j * s] = mk(ujj, 0.0);
+        for (int i = j + 1; i < s; ++i) {
+            // U[j,i] = (G[j,i] - sum_k conj(U[k,j]) U[k,i]) / U[j,j]
+            c64 v = G[j + (size_t)i * s];
+            for (int k = 0; k < j; ++k) {
+                const c64 a = U[k + (size_t)j * s], b = U[k + (size_t)i * s];
+                v.x -= a.x * b.x + a.y * b.y;
+                v.y -= a.x * b.y - a.y * b.x;
+            }
+            U[j + (size_t)i * s] = mk(v.x / ujj, v.y / ujj);
+        }
+    }
+    return true;
+}
+
+static HostMat upper_inverse(const HostMat& U, int s)
+{
+    HostMat X((size_t)s * s, mk(0.0, 0.0));
+    for (int j = 0; j < s; ++j) {
+        // solve U x = e_j (back substitution)
+        for (int i = j; i >= 0; --i) {
+            c64 v = (i == j) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+            for (int k = i + 1; k <= j; ++k) {
+                const c64 a = U[i + (size_t)k * s], b = X[k + (size_t)j * s];
+                v.x -= a.x * b.x - a.y * b.y;
+                v.y -= a.x * b.y + a.y * b.x;
+            }
+            const c64 d = U[i + (size_t)i * s];
+            const double den = d.x * d.x + d.y * d.y;
+            X[i + (size_t)j * s] = mk((v.x * d.x + v.y * d.y) / den, (v.y * d.x - v.x * d.y) / den);
+        }
+    }
+    return X;
+}
+
+template <typename R>
+int qr_fast(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s, HostMat& Rm,
+            cplx<R>* scratch)
+{
+    const double floor1 = (sizeof(R) == 8) ? 1e-12 : 1e-9;
+    HostMat G = gram<R>(w, A, lda, s, A, lda, s, rows, 1);
+    HostMat U1, U2;
+    if (!chol_upper(G, s, U1, floor1)) return qr_cgs2<R>(w, A, lda, Q, ldq, rows, s, Rm);
+    HostMat I1 = upper_inverse(U1, s);
+    Combo<R> c1;
+    c1.nm = 1;
+    c1.W[0] = A;
+    c1.ldw[0] = lda;
+    c1.kw[0] = s;
+    c1.M[0] = I1.data();
+    combo<R>(w, c1, scratch, rows, rows, s);
+    HostMat G2 = gram<R>(w, scratch, rows, s, scratch, rows, s, rows, 1);
+    if (!chol_upper(G2, s, U2, 1e-3)) return qr_cgs2<R>(w, A, lda, Q, ldq, rows, s, Rm);
+    HostMat I2 = upper_inverse(U2, s);
+    Combo<R> c2;
+    c2.nm = 1;
+    c2.W[0] = scratch;
+    c2.ldw[0] = rows;
+    c2.kw[0] = s;
+    c2.M[0] = I2.data();
+    combo<R>(w, c2, Q, ldq, rows, s);
+    Rm = hm_mul(U2, U1, s, s, s);
+    return BDDA_OK;
+}
+
+// ------------------------------------------------------------- host algebra
+HostMat hm_mul(const HostMat& A, const HostMat& B, int m, int k, int n)
+{
+    HostMat C((size_t)m * n, mk(0.0, 0.0));
+    for (int j = 0; j < n; ++j)
+        for (int l = 0; l < k; ++l) {
+            const c64 b = B[l + (size_t)j * k];
+            for (int i = 0; i < m; ++i) {
+                const c64 a = A[i + (size_t)l * m];
+                C[i + (size_t)j * m].x += a.x * b.x - a.y * b.y;
+                C[i + (size_t)j * m].y += a.x * b.y + a.y * b.x;
+            }
+        }
+    return C;
+}
+
+HostMat hm_tmul(const HostMat& A, const HostMat& B, int k, int m, int n)
+{
+    HostMat C((size_t)m * n, mk(0.0, 0.0));
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) {
+            c64 s = mk(0.0, 0.0);
+            for (int l = 0; l < k; ++l) {
+                const c64 a = A[l + (size_t)i * k], b = B[l + (size_t)j * k];
+                s.x += a.x * b.x - a.y * b.y;
+                s.y += a.x * b.y + a.y * b.x;
+            }
+            C[i + (size_t)j * m] = s;
+        }
+    return C;
+}
+
+double hm_fro(const HostMat& A)
+{
+    double s = 0;
+    for (const auto& v : A) s += v.x * v.x + v.y * v.y;
+    return std::sqrt(s);
+}
+
+bool hm_lu_solve(const HostMat& A_, const HostMat& B, int s, int nb, int full, HostMat& X)
+{
+    HostMat A = A_;
+    std::vector<int> rp(s), cp(s);
+    for (int i = 0; i < s; ++i) rp[i] = cp[i] = i;
+    auto abs2 = [](c64 v) { return std::sqrt(v.x * v.x + v.y * v.y); };
+    for (int k = 0; k < s; ++k) {
+        int pr = k, pc = k;
+        double best = -1;
+        for (int j = k; j < (full ? s : k + 1); ++j)
+            for (int i = k; i < s; ++i) {
+                const double v = abs2(A[i + (size_t)j * s]);
+                if (v > best) { best = v; pr = i; pc = j; }
+            }
+        if (pr != k) {
+            for (int j = 0; j < s; ++j) std::swap(A[k + (size_t)j * s], A[pr + (size_t)j * s]);
+            std::swap(rp[k], rp[pr]);
+        }
+        if (pc != k) {
+            for (int i = 0; i < s; ++i) std::swap(A[i + (size_t)k * s], A[i + (size_t)pc * s]);
+            std::swap(cp[k], cp[pc]);
+        }
+        const c64 piv = A[k + (size_t)k * s];
+        const double den = piv.x * piv.x + piv.y * piv.y;
+        if (!(den > 0)) return false;
+        for (int i = k + 1; i < s; ++i) {
+            const c64 a = A[i + (size_t)k * s];
+            const c64 l = mk((a.x * piv.x + a.y * piv.y) / den, (a.y * piv.x - a.x * piv.y) / den);
+            A[i + (size_t)k * s] = l;
+            for (int j = k + 1; j < s; ++j) {
+                const c64 u = A[k + (size_t)j * s];
+                A[i + (size_t)j * s].x -= l.x * u.x - l.y * u.y;
+                A[i + (size_t)j * s].y -= l.x * u.y + l.y * u.x;
+            }
+        }
+    }
+    X.assign((size_t)s * nb, mk(0.0, 0.0));
+    std::vector<c64> y(s);
+    for (int c = 0; c < nb; ++c) {
+        for (int i = 0; i < s; ++i) y[i] = B[rp[i] + (size_t)c * s];
+        for (int i = 0; i < s; ++i)
+            for (int j = 0; j < i; ++j) {
+                const c64 a = A[i + (size_t)j * s];
+                y[i].x -= a.x * y[j].x - a.y * y[j].y;
+                y[i].y -= a.x * y[j].y + a.y * y[j].x;
+            }
+        for (int i = s - 1; i >= 0; --i) {
+            for (int j = i + 1; j < s; ++j) {
+                const c64 a = A[i + (size_t)j * s];
+                y[i].x -= a.x * y[j].x - a.y * y[j].y;
+                y[i].y -= a.x * y[j].y + a.y * y[j].x;
+            }
+            const c64 d = A[i + (size_t)i * s];
+            const double den = d.x * d.x + d.y * d.y;
+            y[i] = mk((y[i].x * d.x + y[i].y * d.y) / den, (y[i].y * d.x - y[i].x * d.y) / den);
+        }
+        for (int i = 0; i < s; ++i) X[cp[i] + (size_t)c * s] = y[i];
+    }
+    return true;
+}
+
+// ------------------------------------------------------------- instantiations
+#define BDDA_INST(R)                                                                                          \
+    template HostMat gram<R>(BlockWork&, const cplx<R>*, int64_t, int, const cplx<R>*, int64_t, int, int64_t,  \
+                             int);                                                                             \
+    template c64 dot_all<R>(BlockWork&, const cplx<R>*, int64_t, const cplx<R>*, int64_t, int64_t, int, int); \
+    template double norm_fro<R>(BlockWork&, const cplx<R>*, int64_t, int64_t, int);                            \
+    template void combo<R>(BlockWork&, const Combo<R>&, cplx<R>*, int64_t, int64_t, int);                      \
+    template void scale_rows<R>(hipStream_t, const cplx<R>*, const cplx<R>*, int64_t, cplx<R>*, int64_t,       \
+                                int64_t, int);                                                                 \
+    template void fill_zero<R>(hipStream_t, cplx<R>*, int64_t, int64_t, int);                                  \
+    template int qr_cgs2<R>(BlockWork&, const cplx<R>*, int64_t, cplx<R>*, int64_t, int64_t, int, HostMat&);   \
+    template int qr_fast<R>(BlockWork&, const cplx<R>*, int64_t, cplx<R>*, int64_t, int64_t, int, HostMat&, \
+                            cplx<R>*);
+BDDA_INST(float)
+BDDA_INST(double)
+
+}  // namespace bdda

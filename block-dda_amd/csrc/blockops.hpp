@@ -1,0 +1,98 @@
+// blockops.hpp -- block-vector kernels for the block Krylov solvers.
+//
+// A block vector is rows x s complex values, column-major with leading
+// dimension ld (the layout of Eigen::MatrixXcd in the reference solvers,
+// e.g. bl_cocg_rq_mvp_fft.cpp:23-35).  Reductions accumulate in FP64 for
+// either storage precision; small s x s results come back to the host in
+// FP64 where the reference's Eigen LU / products run (host side, FP64).
+#pragma once
+#include "bdda_common.hpp"
+#include <vector>
+
+namespace bdda {
+
+using HostMat = std::vector<c64>;  // column-major s x s (or a x b)
+
+struct BlockWork {
+    hipStream_t stream = nullptr;
+    c64* partial = nullptr;    // device partial sums
+    size_t partial_elems = 0;
+    c64* red = nullptr;        // device reduced result
+    size_t red_elems = 0;
+    c64* coef = nullptr;       // device coefficient staging (up to 4 x 64 x 64)
+    c64* host_pinned = nullptr;
+    size_t host_elems = 0;
+    int coef_slot = 0;         // rotating staging slot for combo coefficients
+    ~BlockWork();
+    void reserve(size_t partial_n, size_t red_n);
+};
+
+// C(a x b) = op(A)^ B with op = transpose (conjA = 0) or adjoint (conjA = 1),
+// A rows x a (lda), B rows x b (ldb).  Result on the host in FP64.
+template <typename R>
+HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* B, int64_t ldb, int b,
+             int64_t rows, int conjA);
+
+// sum over all entries of conj(A) .* B (conjA = 1) or A .* B; rows x s
+template <typename R>
+c64 dot_all(BlockWork& w, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb, int64_t rows, int s,
+            int conjA);
+
+template <typename R>
+double norm_fro(BlockWork& w, const cplx<R>* A, int64_t lda, int64_t rows, int s);
+
+// Z = sum_t a_t V_t + sum_u W_u M_u  (M_u: host FP64, k_u x s, column-major).
+// Z may alias a V_t but never a W_u.
+template <typename R>
+struct Combo {
+    int nv = 0;
+    const cplx<R>* V[3];
+    int64_t ldv[3];
+    c64 a[3];
+    int nm = 0;
+    const cplx<R>* W[2];
+    int64_t ldw[2];
+    int kw[2];
+    const c64* M[2];  // host pointers
+};
+template <typename R>
+void combo(BlockWork& w, const Combo<R>& c, cplx<R>* Z, int64_t ldz, int64_t rows, int s);
+
+// elementwise: Z[:, j] = d .* A[:, j]  (d has `rows` entries)
+template <typename R>
+void scale_rows(hipStream_t st, const cplx<R>* d, const cplx<R>* A, int64_t lda, cplx<R>* Z, int64_t ldz,
+                int64_t rows, int s);
+
+// precision conversions (block vectors)
+void convert_64_to_32(hipStream_t st, const c64* A, int64_t lda, c32* Z, int64_t ldz, int64_t rows, int s,
+                      double scale);
+void accumulate_32_to_64(hipStream_t st, const c32* A, int64_t lda, c64* Z, int64_t ldz, int64_t rows, int s,
+                         double scale);
+
+template <typename R>
+void fill_zero(hipStream_t st, cplx<R>* Z, int64_t ldz, int64_t rows, int s);
+
+// reduced QR with first-pass R semantics (qr_reduced.cpp:9-31), column-
+// sequential classical Gram-Schmidt with one re-orthogonalisation pass.
+// Q may alias A.  Returns BDDA_OK or BDDA_EBREAKDOWN (a zero column).
+template <typename R>
+int qr_cgs2(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s,
+            HostMat& Rm);
+
+// reduced QR by Cholesky QR applied twice (two Gram + two update sweeps);
+// equal in exact arithmetic to the MGS factorisation with positive real
+// diag(R).  Falls back to qr_cgs2 when a Cholesky pivot fails.  scratch is
+// rows x s (ld = rows) and must not alias A or Q; Q may alias A.
+template <typename R>
+int qr_fast(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s,
+            HostMat& Rm, cplx<R>* scratch);
+
+// ---------------- small dense FP64 algebra on the host (s <= 64)
+HostMat hm_mul(const HostMat& A, const HostMat& B, int m, int k, int n);       // A (m x k) B (k x n)
+HostMat hm_tmul(const HostMat& A, const HostMat& B, int k, int m, int n);      // A^T (A: k x m) B (k x n)
+// X = A^{-1} B, partial (full = 0) or full (full = 1) pivoting; returns
+// false when a pivot is exactly zero
+bool hm_lu_solve(const HostMat& A, const HostMat& B, int s, int nb, int full, HostMat& X);
+double hm_fro(const HostMat& A);
+
+}  // namespace bdda

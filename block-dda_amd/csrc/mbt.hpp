@@ -1,0 +1,81 @@
+// mbt.hpp -- the MBT (multilevel block-Toeplitz) operator on one GPU.
+//
+// The reference evaluates y_i = sum_j M(lf (i - j)) x_j over the nx*ny*nz
+// cuboid through one 1-D FFT of length f^2 (2nx-1)(2ny-1)(2nz-1)
+// (mvp_fft/MBT_fft_mvp.cpp:8-43).  Here the same linear convolution is a
+// zero-padded 3-D FFT convolution on a Px*Py*Pz grid (P >= 2n-1 per axis,
+// 2/3/5/7-smooth) with a per-frequency f x f multiply by the transformed
+// interaction kernel G^ (symmetry-reduced to one octant).
+//
+// Device layouts (complex T interleaved, component index innermost):
+//   occupied vectors   X[rhs * ld + f m + c]                (reference layout)
+//   U  (after z pass)  U[((rhs nx + ix) ny + iy) Pz + kz][c]
+//   V  (after y pass)  V[((rhs nx + ix) Py + ky) Pz + kz][c]
+//   G^ reduced         Gh[((kx' (Py/2+1) + ky') (Pz/2+1) + kz')][nc]
+//                      nc = 6 (f=3: xx yy zz xy xz yz of -G)
+//                      nc = 9 (f=6: the same + xy xz yz of -F)
+#pragma once
+#include "bdda_common.hpp"
+#include "fft_engine.hpp"
+#include <vector>
+
+namespace bdda {
+
+struct MbtDims {
+    int nx, ny, nz, f;
+    int Px, Py, Pz;
+};
+
+// choose the padded FFT size for one axis: smallest 2/3/5/7-smooth
+// P >= 2n-1, a multiple of `mult`
+int choose_padded(int n, int mult);
+AxisPlan make_axis_plan(int P, bool last8);
+
+enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1 };
+
+struct MbtOperator {
+    MbtDims d;
+    double lf;
+    double kre, kim;
+    AxisPlan ax[3];             // x plan ends with radix 8
+    void* tw64[3] = {nullptr, nullptr, nullptr};  // per-axis twiddle tables
+    void* tw32[3] = {nullptr, nullptr, nullptr};
+    void* gh64 = nullptr;       // reduced G^, FP64 (scaled by 1/(PxPyPz))
+    void* gh32 = nullptr;       // reduced G^, FP32 copy
+    int have64 = 0, have32 = 0;
+    int nc = 6;
+    size_t gh_elems = 0;        // reduced elements per component set
+    hipStream_t stream = nullptr;
+    // workspace for the passes (grown on demand)
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    // occupancy map for the occupied-vector path
+    int* c2o = nullptr;         // cell -> occupied index or -1
+    int64_t c2o_cells = 0;
+    std::vector<int> c2o_host_addr;  // address array the map was built from
+    // launch timing of the last apply (ms per pass), filled when timing on
+    int timing = 0;
+    hipEvent_t ev[8] = {};
+    float pass_ms[6] = {};
+};
+
+// Create the operator: padded sizes, plans, G^ build (FP64 always, FP32
+// copy when want32).  Throws HipError on runtime failure.
+MbtOperator* mbt_create(const int n[3], int f, double lf, double kre, double kim,
+                        int want64, int want32, hipStream_t stream);
+void mbt_destroy(MbtOperator* op);
+
+// set the occupancy (address[m] = ny nz ix + nz iy + iz, 0-based)
+void mbt_set_address(MbtOperator* op, const int* address_host, int64_t nocc);
+
+// Y[:, r] = diag .* X[:, r] + gather(conv(scatter(X[:, r])))  (then .* jpre)
+// for r < s, all pointers on the device.  diag/jpre may be null.
+template <typename R>
+void mbt_apply_occupied(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int64_t ldy,
+                        int s, const cplx<R>* diag, const cplx<R>* jpre, int64_t nocc);
+
+// bytes per block matvec under the minimal fused pipeline (the algorithmic
+// model used for roofline numbers)
+double mbt_algorithmic_bytes(const MbtOperator* op, int s, int64_t nocc, size_t elem_bytes);
+
+}  // namespace bdda

@@ -1,0 +1,144 @@
+/*
+ * bdda.h -- C ABI of the MI355X (gfx950) block-DDA solve path.
+ *
+ * The reference exposes this path as free C++ functions over Eigen types and
+ * FFTW plans (mvp_fft/MBT_utils.hpp:1-6, DDA_linear_solvers.hpp:1-6,
+ * linear_algebra_addon.hpp:1).  This header carries the same operations over
+ * plain pointers so that any host language can bind them:
+ *   - complex values are interleaved (re, im) doubles: the memory layout of
+ *     std::complex<double>, Eigen::VectorXcd::data(), numpy complex128;
+ *   - matrices are column-major with an explicit leading dimension (Eigen's
+ *     default MatrixXcd layout);
+ *   - the cuboid vector layout is the reference's: element (ix, iy, iz, q)
+ *     at f*(ny*nz*ix + nz*iy + iz) + q; occupied vectors are (f*N) x L with
+ *     dipole m at rows f*m .. f*m+f-1, address[m] = ny*nz*ix + nz*iy + iz
+ *     (0-based, Target_manager.cpp:98,167-169);
+ *   - every entry point returns a status code (BDDA_OK = 0) and never exits;
+ *     bdda_last_error() describes the last failure on the calling thread.
+ * One handle = one HIP stream; calls on the same handle must not overlap.
+ */
+#ifndef BDDA_H
+#define BDDA_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double re, im;
+} bdda_c64;
+
+typedef struct bdda_mbt bdda_mbt;
+
+enum {
+    BDDA_OK = 0,
+    BDDA_EINVAL = 1,
+    BDDA_ENOMEM = 2,
+    BDDA_EHIP = 3,
+    BDDA_EBREAKDOWN = 4,
+    BDDA_ENOCONV = 5,
+    BDDA_ENAN = 6,
+    BDDA_EUNSUPPORTED = 7
+};
+
+/* precision masks / modes */
+enum { BDDA_PREC_FP64 = 1, BDDA_PREC_FP32 = 2 };
+enum { BDDA_MODE_FP64 = 0, BDDA_MODE_FP32 = 1, BDDA_MODE_MIXED = 2 };
+enum { BDDA_COCG_RQ = 0, BDDA_BICGSTAB = 1, BDDA_BICGGR = 2 };
+
+typedef struct {
+    int iters;             /* block iterations (summed over defect-correction steps) */
+    int outer_iters;       /* defect-correction steps (1 for FP64 / FP32 modes) */
+    int matvecs;           /* block matvecs (each covers all L columns) */
+    int converged;         /* 1 when the stopping test was met */
+    int status;            /* BDDA_* of the solve */
+    double recurrence_res; /* last residual as the reference computes it */
+    double true_res;       /* ||B - A X||_F / ||B||_F in FP64 (-1 if not computed) */
+    double time_ms;        /* wall time of the solve on the host clock */
+} bdda_solve_info;
+
+const char* bdda_last_error(void);
+int bdda_version(void);
+
+/* ---------------------------------------------------------------- layout
+ * CPU reference-layout helpers (no GPU needed).
+ * application_function (mvp_fft/application_function.cpp:7-78): f x f
+ *   column-major block for the 2x3 index matrix n_ind (row-major: row 0 =
+ *   n_ind[0..2], row 1 = n_ind[3..5]).
+ * BT_fft (mvp_fft/BT_fft.cpp:8-53): Au, length f^2 prod(2n-1).
+ * BT_pad (mvp_fft/BT_pad.cpp:6-45): xz, length f^2 nx (2ny-1)(2nz-1).
+ * BT_reconstruct (mvp_fft/BT_reconstruct.cpp:5-36): b, length f nx ny nz. */
+int64_t bdda_fft_length(const int n[3], int f);
+int bdda_application_function(const int n_ind[6], int f, double lf, bdda_c64 k, bdda_c64* out);
+int bdda_bt_fft(const int n[3], int f, double lf, bdda_c64 k, bdda_c64* Au);
+int bdda_bt_pad(const int n[3], int f, const bdda_c64* x, bdda_c64* xz);
+int bdda_bt_reconstruct(const int n[3], int f, const bdda_c64* bz, bdda_c64* b);
+
+/* ---------------------------------------------------------------- operator
+ * bdda_mbt_create replaces Solver::fft_prep + MBT_fft_init
+ * (Solver.cpp:17-25,55; mvp_fft/MBT_fft_init.cpp:10-42): it builds the
+ * transformed interaction kernel on the device.  prec_mask selects which
+ * precisions (BDDA_PREC_*) the operator can apply.  f is 3 or 6. */
+int bdda_mbt_create(const int n[3], int f, double lf, bdda_c64 k, int prec_mask, bdda_mbt** out);
+void bdda_mbt_destroy(bdda_mbt* h);
+/* padded FFT grid chosen for the operator */
+int bdda_mbt_grid(const bdda_mbt* h, int P[3]);
+
+/* MBT_fft_mvp (mvp_fft/MBT_fft_mvp.cpp:8-43) on ncols cuboid vectors
+ * (host buffers, f*nx*ny*nz rows each): y = offdiag(A) x.  prec is
+ * BDDA_PREC_FP64 or BDDA_PREC_FP32 (inputs/outputs stay complex<double>). */
+int bdda_mbt_apply(bdda_mbt* h, int prec, const bdda_c64* x, int64_t ldx, bdda_c64* y, int64_t ldy, int ncols);
+
+/* Occupancy and DIAG_A of the target (Solver.cpp:28-47: DIAG_A = 1/alpha
+ * per component).  diag has f*nocc entries. */
+int bdda_mbt_set_target(bdda_mbt* h, const int32_t* address, int64_t nocc, const bdda_c64* diag);
+
+/* The inline block matvec every reference solver repeats
+ * (bl_cocg_rq_jacobi_mvp_fft.cpp:43-60): Y = DIAG_A .* X + gather(MBT(
+ * scatter(X))), then Y = jpre .* Y with jpre = 1/DIAG_A when jacobi != 0. */
+int bdda_mbt_apply_occupied(bdda_mbt* h, int prec, int jacobi, const bdda_c64* X, int64_t ldx, bdda_c64* Y,
+                            int64_t ldy, int ncols);
+
+/* Block solvers (DDA_linear_solvers.hpp:1-6):
+ *   method BDDA_COCG_RQ  + jacobi 0/1 -> bl_cocg_rq_mvp_fft / bl_cocg_rq_jacobi_mvp_fft
+ *   method BDDA_BICGSTAB + jacobi 0/1 -> bl_bicgstab_mvp_fft / ..._jacobi_...
+ *   method BDDA_BICGGR   + jacobi 0/1 -> bl_bicggr_mvp_fft / ..._jacobi_...
+ * mode BDDA_MODE_FP64 keeps the reference's FP64 arithmetic and stopping
+ * rule (recurrence residual < tol); BDDA_MODE_MIXED runs the inner block
+ * iteration in complex FP32 inside an FP64 defect-correction loop that stops
+ * on the true FP64 residual ||B - A X||/||B|| <= tol.  hist (optional,
+ * hist_len entries) receives per-iteration residuals (outer true residuals
+ * are stored negated).  Returns BDDA_ENOCONV when itermax was reached. */
+int bdda_solve(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64* B, int64_t ldb, int L, double tol,
+               int itermax, bdda_c64* X, int64_t ldx, bdda_solve_info* info, double* hist, int hist_len);
+
+/* qr_reduced (bl_krylov_mvp_fft/qr_reduced.cpp:9-31) on the GPU: A is m x n
+ * column-major (ld = m); Q (m x n) and R (n x n, R(i,k) first-pass
+ * coefficients, R(k,k) the norm after re-orthogonalisation). */
+int bdda_qr_reduced(const bdda_c64* A, int64_t m, int n, bdda_c64* Q, bdda_c64* R);
+
+/* ---------------------------------------------------------------- device
+ * Device-resident variants (pointers from bdda_dev_alloc, complex<double>
+ * layout) for callers that keep B and X in HBM. */
+int bdda_dev_alloc(size_t bytes, void** p);
+int bdda_dev_free(void* p);
+int bdda_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int bdda_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int bdda_device_sync(void);
+int bdda_solve_dev(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64* dB, int L, double tol,
+                   int itermax, bdda_c64* dX, bdda_solve_info* info, double* hist, int hist_len);
+/* device block matvec in the given precision; dX/dY hold complex<float>
+ * (prec FP32) or complex<double> (prec FP64), ld = f*nocc */
+int bdda_mbt_apply_occupied_dev(bdda_mbt* h, int prec, int jacobi, const void* dX, void* dY, int ncols);
+/* per-pass kernel timing of the following applies (HIP events on the
+ * handle's stream); pass_ms receives z-fwd, y-fwd, x-fused, y-inv, z-inv */
+int bdda_mbt_set_timing(bdda_mbt* h, int on);
+int bdda_mbt_pass_times(const bdda_mbt* h, float pass_ms[5]);
+/* algorithmic HBM bytes of one block matvec of ncols columns */
+double bdda_mbt_algorithmic_bytes(const bdda_mbt* h, int prec, int ncols);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

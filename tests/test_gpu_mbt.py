@@ -1,0 +1,125 @@
+"""GPU parity of the MBT operator (HIP library through the C ABI) against
+the oracle: the dense block-Toeplitz product on tiny grids, the oracle's
+1-D FFT restatement of MBT_fft_mvp on medium grids, and size-independent
+properties (linearity, complex symmetry, embedding independence) on larger
+grids.  Tolerances: 1e-12 relative in FP64, 1e-5 in FP32."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL64, TOL32 = 1e-12, 1e-5
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def rnd(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("f", [3, 6])
+@pytest.mark.parametrize("n", [(2, 3, 4), (1, 3, 2), (5, 4, 5), (4, 4, 3), (6, 5, 7), (1, 1, 1)])
+def test_mvp_dense(bdda, orc, n, f):
+    print("case", n, f, flush=True)
+    rng = np.random.default_rng(1)
+    lf, k = 0.7, 1.3 + 0.05j
+    op = bdda.MBT_fft_init(n, f, lf, k)
+    x = rnd(rng, f * int(np.prod(n)))
+    ref = orc.dense_offdiag(n, f, lf, k) @ x
+    if not np.abs(ref).max():
+        assert not np.abs(bdda.MBT_fft_mvp(n, f, op, x)).max()
+        return
+    assert rel(bdda.MBT_fft_mvp(n, f, op, x), ref) <= TOL64
+    assert rel(op.apply(x, "fp32"), ref) <= TOL32
+
+
+@pytest.mark.parametrize("f", [3, 6])
+@pytest.mark.parametrize("n", [(12, 10, 9), (17, 9, 8), (9, 16, 15)])
+def test_mvp_oracle_medium(bdda, orc, n, f):
+    print("case", n, f, flush=True)
+    rng = np.random.default_rng(2)
+    lf, k = 0.05, 2.1 + 0.0j
+    op = bdda.MBT_fft_init(n, f, lf, k)
+    Au = orc.mbt_fft_init(n, f, lf, k)
+    X = rnd(rng, f * int(np.prod(n)), 2)
+    Y = op.apply(X)
+    for j in range(2):
+        ref = orc.mbt_fft_mvp(n, f, Au, X[:, j])
+        assert rel(Y[:, j], ref) <= TOL64
+    assert rel(op.apply(X, "fp32"), Y) <= TOL32
+
+
+@pytest.mark.parametrize("f", [3, 6])
+def test_occupied_matvec_vs_oracle(bdda, orc, f):
+    rng = np.random.default_rng(3)
+    n = (7, 6, 5)
+    C = int(np.prod(n))
+    addr = np.sort(rng.choice(C, size=120, replace=False)).astype(np.int32)
+    alpha = rng.standard_normal(120) * 0.1 + 0.3 + 0.05j
+    diag = np.repeat(1.0 / alpha, f)
+    lf, k = 0.4, 1.7
+    op = bdda.MBT_fft_init(n, f, lf, k)
+    Au = orc.mbt_fft_init(n, f, lf, k)
+    op.set_target(addr, diag)
+    X = rnd(rng, f * addr.size, 3)
+    for jac in (False, True):
+        ref = orc.block_matvec(n, f, addr, Au, diag, X, jacobi=jac)
+        assert rel(op.apply_occupied(X, "fp64", jac), ref) <= TOL64
+        assert rel(op.apply_occupied(X, "fp32", jac), ref) <= TOL32
+    # unsorted address order must give the same products
+    perm = rng.permutation(addr.size)
+    op2 = bdda.MBT_fft_init(n, f, lf, k)
+    op2.set_target(addr[perm], np.repeat(1.0 / alpha[perm], f))
+    Xp = X.reshape(addr.size, f, 3)[perm].reshape(-1, 3)
+    Yp = op2.apply_occupied(Xp)
+    ref = orc.block_matvec(n, f, addr, Au, diag, X)
+    assert rel(Yp, ref.reshape(addr.size, f, 3)[perm].reshape(-1, 3)) <= TOL64
+
+
+def test_properties_large(bdda):
+    """Linearity and complex symmetry (f=3: x^T A y = y^T A x) on 48x40x36."""
+    rng = np.random.default_rng(4)
+    n = (48, 40, 36)
+    op = bdda.MBT_fft_init(n, 3, 0.02, 3.0 + 0.01j)
+    N = 3 * int(np.prod(n))
+    x, y = rnd(rng, N), rnd(rng, N)
+    a, b = 0.3 - 1.1j, 2.0 + 0.5j
+    Ax, Ay, Axy = op.apply(np.stack([x, y, a * x + b * y], axis=1)).T
+    assert rel(Axy, a * Ax + b * Ay) <= TOL64
+    assert abs(x @ Ay - y @ Ax) <= TOL64 * abs(x @ Ay) * 10
+    Ax32 = op.apply(x, "fp32")
+    assert rel(Ax32, Ax) <= TOL32
+
+
+@pytest.mark.parametrize("f", [3, 6])
+def test_embedding_independence(bdda, f):
+    """A zero layer added to the cuboid changes the padded grid but not the product."""
+    rng = np.random.default_rng(5)
+    n = (9, 7, 6)
+    n2 = (11, 7, 8)
+    x = rnd(rng, *n, f)
+    x2 = np.zeros(n2 + (f,), dtype=complex)
+    x2[: n[0], :, : n[2]] = x
+    op = bdda.MBT_fft_init(n, f, 0.3, 1.4)
+    op2 = bdda.MBT_fft_init(n2, f, 0.3, 1.4)
+    assert op.grid != op2.grid
+    y = op.apply(x.ravel()).reshape(n + (f,))
+    y2 = op2.apply(x2.ravel()).reshape(n2 + (f,))
+    assert rel(y2[: n[0], :, : n[2]], y) <= TOL64
+
+
+def test_operator_errors(bdda):
+    with pytest.raises(bdda.BddaError) as e:
+        bdda.MBT_fft_init((4, 4, 4), 4, 1.0, 1.0)
+    assert e.value.code == 1
+    with pytest.raises(bdda.BddaError):
+        bdda.MBT_fft_init((0, 4, 4), 3, 1.0, 1.0)
+    op = bdda.MBT_fft_init((4, 4, 4), 3, 1.0, 1.0)
+    with pytest.raises(bdda.BddaError):
+        op.set_target(np.array([0, 64], dtype=np.int32), np.ones(6))   # outside the cuboid
+    with pytest.raises(bdda.BddaError):
+        op.set_target(np.array([3, 3], dtype=np.int32), np.ones(6))    # duplicate address
+    with pytest.raises(bdda.BddaError):
+        op.apply_occupied(np.ones(6))                                   # no target yet

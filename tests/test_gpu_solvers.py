@@ -1,0 +1,124 @@
+"""GPU parity of the six block solvers, qr_reduced and the mixed-precision
+defect correction against the oracle (C restatement of the reference
+solvers) and the dense direct solve; plus the error paths the reference
+leaves undefined (NaN input, rank-deficient block, non-convergence)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = {
+    ("cocg_rq", False): "bl_cocg_rq_mvp_fft",
+    ("cocg_rq", True): "bl_cocg_rq_jacobi_mvp_fft",
+    ("bicgstab", False): "bl_bicgstab_mvp_fft",
+    ("bicgstab", True): "bl_bicgstab_jacobi_mvp_fft",
+    ("bicggr", False): "bl_bicggr_mvp_fft",
+    ("bicggr", True): "bl_bicggr_jacobi_mvp_fft",
+}
+
+
+def problem(bdda, f, n=(6, 5, 7), nocc=150, L=4, seed=1, k=0.9, lf=0.5, two_materials=True):
+    """Random occupancy with LDR polarizabilities of one or two materials."""
+    T = targets_mod(bdda)
+    rng = np.random.default_rng(seed)
+    C = int(np.prod(n))
+    addr = np.sort(rng.choice(C, size=nocc, replace=False)).astype(np.int32)
+    alpha = np.full(nocc, T.alpha_ldr(1.5 + 0.1j, lf, k), dtype=complex)
+    if two_materials:  # Jacobi then differs from the plain solver
+        alpha[rng.random(nocc) < 0.3] = T.alpha_ldr(2.0 + 0.5j, lf, k)
+    diag = np.repeat(1.0 / alpha, f)
+    B = rng.standard_normal((f * nocc, L)) + 1j * rng.standard_normal((f * nocc, L))
+    return n, addr, diag, B, lf, k
+
+
+def targets_mod(bdda):
+    import importlib
+    return importlib.import_module("block_dda_amd.targets")
+
+
+@pytest.mark.parametrize("method,jacobi", list(SOLVERS))
+@pytest.mark.parametrize("f", [3, 6])
+def test_solver_fp64_parity(bdda, orc, method, jacobi, f):
+    if method == "cocg_rq" and f == 6:
+        pytest.skip("COCG needs a complex-symmetric matrix (f = 3 only)")
+    print("case", method, jacobi, f, flush=True)
+    # left Jacobi scaling keeps D^-1 A complex symmetric only for one material,
+    # so COCG-rq-Jacobi is exercised on a single-material target
+    n, addr, diag, B, lf, k = problem(bdda, f, two_materials=not (method == "cocg_rq" and jacobi))
+    tol, itmax = 1e-9, 500
+    Au = orc.mbt_fft_init(n, f, lf, k)
+    Xo, ito, ho = orc.solve(method, n, f, addr, Au, diag, B, tol, itmax, jacobi=jacobi)
+    op = bdda.MBT_fft_init(n, f, lf, k)
+    fn = getattr(bdda, SOLVERS[(method, jacobi)])
+    X, info = fn(n, f, addr, op, diag, B, tol, itmax, return_info=True)
+    assert info["converged"] == 1
+    # the residual history follows the oracle's while rounding is small
+    op.set_target(addr, diag)
+    _, _, hist = op.solve(B, tol, itmax, method=bdda.__dict__[method.upper()], jacobi=jacobi, hist_len=itmax)
+    m = min(5, ito)
+    assert np.allclose(hist[:m], ho[:m], rtol=1e-8, atol=0)
+    assert abs(info["iters"] - ito) <= max(3, 0.15 * ito)
+    # both are solutions of the same system
+    A = orc.dense_system(n, f, lf, k, addr, diag)
+    assert np.linalg.norm(A @ X - B) / np.linalg.norm(B) < 50 * tol
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-6
+    assert info["true_res"] < 50 * tol
+
+
+@pytest.mark.parametrize("method", ["cocg_rq", "bicgstab", "bicggr"])
+def test_mixed_precision_reaches_fp64_accuracy(bdda, orc, method):
+    print("case mixed", method, flush=True)
+    n, addr, diag, B, lf, k = problem(bdda, 3, seed=4)
+    op = bdda.MBT_fft_init(n, 3, lf, k)
+    op.set_target(addr, diag)
+    tol = 1e-9
+    X, info, _ = op.solve(B, tol, 600, method=bdda.__dict__[method.upper()], mode=bdda.MODE_MIXED)
+    assert info["converged"] == 1 and info["outer_iters"] >= 2
+    A = orc.dense_system(n, 3, lf, k, addr, diag)
+    res = np.linalg.norm(A @ X - B) / np.linalg.norm(B)
+    assert res <= tol * 1.01 and abs(info["true_res"] - res) <= 0.05 * res + 1e-15
+    Xd = np.linalg.solve(A, B)
+    assert np.linalg.norm(X - Xd) / np.linalg.norm(Xd) < 1e-7
+
+
+def test_fp32_mode_is_fp32_accurate(bdda, orc):
+    n, addr, diag, B, lf, k = problem(bdda, 3, seed=6)
+    op = bdda.MBT_fft_init(n, 3, lf, k)
+    op.set_target(addr, diag)
+    X, info, _ = op.solve(B, 1e-5, 400, mode=bdda.MODE_FP32)
+    assert info["converged"] == 1
+    A = orc.dense_system(n, 3, lf, k, addr, diag)
+    assert np.linalg.norm(A @ X - B) / np.linalg.norm(B) < 1e-4
+
+
+def test_qr_reduced_gpu(bdda, orc):
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((3000, 8)) + 1j * rng.standard_normal((3000, 8))
+    Q, R = bdda.qr_reduced(A)
+    Qo, Ro = orc.qr_reduced(A)
+    assert np.abs(Q - Qo).max() < 1e-12 and np.abs(R - Ro).max() < 1e-12 * np.abs(Ro).max()
+    assert np.abs(np.tril(R, -1)).max() == 0 and (np.diag(R).real > 0).all()
+    A[:, 3] = A[:, 1]
+    with pytest.raises(bdda.BddaError) as e:
+        bdda.qr_reduced(A)
+    assert e.value.code == 4
+
+
+def test_solver_error_paths(bdda, orc):
+    n, addr, diag, B, lf, k = problem(bdda, 3, L=3)
+    op = bdda.MBT_fft_init(n, 3, lf, k)
+    op.set_target(addr, diag)
+    Bn = B.copy()
+    Bn[5, 1] = np.nan
+    with pytest.raises(bdda.BddaError) as e:
+        op.solve(Bn, 1e-6, 50)
+    assert e.value.code == 6
+    Bd = B.copy()
+    Bd[:, 2] = Bd[:, 0]          # duplicate right-hand side: rank-deficient block
+    with pytest.raises(bdda.BddaError) as e:
+        op.solve(Bd, 1e-6, 50)
+    assert e.value.code == 4
+    X, info, _ = op.solve(B, 1e-12, 1)   # non-convergence is reported, X returned
+    assert info["converged"] == 0 and info["status"] == 5 and info["iters"] == 1
+    with pytest.raises(bdda.BddaError):
+        op.solve(B, -1.0, 10)
