@@ -99,7 +99,7 @@ template <int P, typename R, bool INV>
 struct DftOdd {
     __device__ __forceinline__ static void run(cplx<R>* v)
     {
-        constexpr TrigTab<P> tab{};
+        static constexpr TrigTab<P> tab{};
         constexpr int H = (P - 1) / 2;
         cplx<R> a[H], b[H];
 #pragma unroll

@@ -1,0 +1,221 @@
+// fft_plan.hpp -- compile-time-planned register-resident Stockham FFT.
+//
+// A work-group of NT threads transforms NQ lines (length P) of NC
+// components; NQ (a power of two) indexes the contiguous axis in HBM, so a
+// wave reading butterfly inputs x[j + r P/RAD][q][c] touches consecutive
+// addresses.  Data stays in registers: stage-0 inputs come straight from
+// the caller's loader, twiddles and the radix-RAD DFT run in registers, and
+// only the exchange between two stages goes through LDS (NCX components per
+// round).  The last stage's outputs sit at positions j + r P/RAD, the same
+// pattern as stage 0's inputs, so they go straight to the caller's storer.
+//
+// Stockham stage (L = product of earlier radices), butterfly j, k = j mod L:
+//   in x[j + r P/RAD] * w_{L RAD}^{r k},   out y[(j - k) RAD + k + r L].
+// Butterfly b = j NQ + q (q fastest); thread t owns b = t + i NT.
+//
+// The radix sequence is a template parameter: every stage's radix, L, the
+// number of butterflies per thread and all strides are compile-time
+// constants, so the register arrays have exact sizes, every index folds and
+// nothing is placed in scratch memory.  Kernels for the padded sizes in the
+// plan table (mbt_plans.hpp) are built on this engine; other sizes use the
+// all-in-LDS engine of fft_engine.hpp.
+#pragma once
+#include <type_traits>
+#include "fft_engine.hpp"
+
+namespace bdda {
+
+template <int... RS>
+struct Radices {
+    static constexpr int N = sizeof...(RS);
+    static constexpr int P = (RS * ... * 1);
+    static constexpr int r[N] = {RS...};
+};
+
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(ic<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// R: real type, PL: Radices<...>, NT threads, NQ lines (power of two), NC
+// components, NCX components per LDS exchange round.  Methods take INV: the
+// inverse transform uses the radices in reverse order, so the forward's last
+// stage and the inverse's first stage share radix and butterfly layout.
+template <typename R, class PL, int NT, int NQ, int NC, int NCX>
+struct PlanFFT {
+    static constexpr int P = PL::P;
+    static constexpr int NS = PL::N;
+    template <int S, bool INV>
+    static constexpr int rad() { return INV ? PL::r[NS - 1 - S] : PL::r[S]; }
+    template <int S, bool INV>
+    static constexpr int Lof()
+    {
+        int L = 1;
+        for (int s = 0; s < S; ++s) L *= INV ? PL::r[NS - 1 - s] : PL::r[s];
+        return L;
+    }
+    template <int S, bool INV>
+    static constexpr int mb() { return (P / rad<S, INV>() * NQ + NT - 1) / NT; }
+    static constexpr int emax()
+    {
+        int m = 0;
+        for (int s = 0; s < NS; ++s) {
+            const int rr = PL::r[s];
+            const int v = ((P / rr * NQ + NT - 1) / NT) * rr;
+            m = v > m ? v : m;
+        }
+        return m;
+    }
+    static constexpr int E = emax();
+    static constexpr int QS = (NQ == 1) ? 0 : (NQ == 2) ? 1 : (NQ == 4) ? 2 : (NQ == 8) ? 3 : (NQ == 16) ? 4 : (NQ == 32) ? 5 : -1;
+    static_assert(QS >= 0, "NQ must be a power of two <= 32");
+
+    cplx<R> v[NC][E];
+    const cplx<R>* tw;  // LDS twiddles, tw[i] = exp(-2 pi i i / P)
+    cplx<R>* lds;       // NCX * P * NQ exchange elements
+
+    __device__ __forceinline__ PlanFFT(const cplx<R>* tw_, cplx<R>* lds_) : tw(tw_), lds(lds_) {}
+
+    // stage-0 inputs: ld(p, q, c)
+    template <bool INV, class LD>
+    __device__ __forceinline__ void load(LD&& ld)
+    {
+        constexpr int RAD = rad<0, INV>(), nb = P / RAD, total = nb * NQ;
+        static_for<0, mb<0, INV>()>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const int b = threadIdx.x + i * NT;
+            if (total % NT == 0 || b < total) {
+                const int q = b & (NQ - 1), j = b >> QS;
+#pragma unroll
+                for (int r = 0; r < RAD; ++r)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) v[c][i * RAD + r] = ld(j + r * nb, q, c);
+            }
+        });
+    }
+
+    // last-stage outputs: st(p, q, c, value)
+    template <bool INV, class ST>
+    __device__ __forceinline__ void store(ST&& st)
+    {
+        constexpr int S = NS - 1, RAD = rad<S, INV>(), nb = P / RAD, total = nb * NQ;
+        static_for<0, mb<S, INV>()>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const int b = threadIdx.x + i * NT;
+            if (total % NT == 0 || b < total) {
+                const int q = b & (NQ - 1), j = b >> QS;
+#pragma unroll
+                for (int r = 0; r < RAD; ++r)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) st(j + r * nb, q, c, v[c][i * RAD + r]);
+            }
+        });
+    }
+
+    // forward last-stage slots: fn(p, q, cplx<R>* vals[NC]) may rewrite vals
+    template <class FN>
+    __device__ __forceinline__ void visit(FN&& fn)
+    {
+        constexpr int S = NS - 1, RAD = rad<S, false>(), nb = P / RAD, total = nb * NQ;
+        static_for<0, mb<S, false>()>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const int b = threadIdx.x + i * NT;
+            if (total % NT == 0 || b < total) {
+                const int q = b & (NQ - 1), j = b >> QS;
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) {
+                    cplx<R> x[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) x[c] = v[c][i * RAD + r];
+                    fn(j + r * nb, q, x);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) v[c][i * RAD + r] = x[c];
+                }
+            }
+        });
+    }
+
+    template <int S, bool INV>
+    __device__ __forceinline__ void compute()
+    {
+        constexpr int RAD = rad<S, INV>(), L = Lof<S, INV>(), nb = P / RAD, total = nb * NQ;
+        constexpr int tstep = P / (L * RAD);
+        static_for<0, mb<S, INV>()>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const int b = threadIdx.x + i * NT;
+            if (total % NT == 0 || b < total) {
+                if constexpr (L > 1) {
+                    const int j = b >> QS;
+                    const int kt = (j % L) * tstep;
+#pragma unroll
+                    for (int r = 1; r < RAD; ++r) {
+                        cplx<R> w = tw[r * kt];
+                        if (INV) w.y = -w.y;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) v[c][i * RAD + r] = v[c][i * RAD + r] * w;
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < NC; ++c) Dft<RAD, R, INV>::run(&v[c][i * RAD]);
+            }
+        });
+    }
+
+    // stage S outputs -> LDS -> stage S+1 inputs, NCX components per round
+    template <int S, bool INV>
+    __device__ __forceinline__ void exchange()
+    {
+        constexpr int RO = rad<S, INV>(), L = Lof<S, INV>(), nbo = P / RO, to = nbo * NQ;
+        constexpr int RI = rad<S + 1, INV>(), nbi = P / RI, ti = nbi * NQ;
+        constexpr int PQ = P * NQ;
+        static_for<0, (NC + NCX - 1) / NCX>([&](auto CB) {
+            constexpr int c0 = decltype(CB)::value * NCX;
+            static_for<0, mb<S, INV>()>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const int b = threadIdx.x + i * NT;
+                if (to % NT == 0 || b < to) {
+                    const int q = b & (NQ - 1), j = b >> QS;
+                    const int k = j % L;
+                    const int base = (((j - k) * RO + k) << QS) + q;
+#pragma unroll
+                    for (int r = 0; r < RO; ++r)
+#pragma unroll
+                        for (int cc = 0; cc < NCX; ++cc)
+                            if (c0 + cc < NC) lds[cc * PQ + base + ((r * L) << QS)] = v[c0 + cc][i * RO + r];
+                }
+            });
+            __syncthreads();
+            static_for<0, mb<S + 1, INV>()>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const int b = threadIdx.x + i * NT;
+                if (ti % NT == 0 || b < ti) {
+#pragma unroll
+                    for (int r = 0; r < RI; ++r)
+#pragma unroll
+                        for (int cc = 0; cc < NCX; ++cc)
+                            if (c0 + cc < NC) v[c0 + cc][i * RI + r] = lds[cc * PQ + b + ((r * nbi) << QS)];
+                }
+            });
+            __syncthreads();
+        });
+    }
+
+    template <bool INV>
+    __device__ __forceinline__ void run()
+    {
+        static_for<0, NS>([&](auto SI) {
+            constexpr int S = decltype(SI)::value;
+            compute<S, INV>();
+            if constexpr (S + 1 < NS) exchange<S, INV>();
+        });
+    }
+};
+
+}  // namespace bdda

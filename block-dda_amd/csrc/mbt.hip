@@ -11,29 +11,13 @@
 // The reference's 1-D FFT of length f^2 prod(2n-1) equals a zero-padded 3-D
 // convolution over the cuboid (no index digit borrows); this file computes
 // that convolution with 3-D FFTs on a smooth padded grid instead.
-#include "mbt.hpp"
+#include "mbt_passes.hpp"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 
 namespace bdda {
-
-// element budget per work-group (elements held in LDS and spread over
-// NT_PASS threads): 24 per thread in FP32, 12 per thread in FP64, so one
-// group uses 48 KB of LDS in either precision and three fit on a CU.
-constexpr int NT_PASS = 256;
-template <typename R>
-struct Budget;
-template <>
-struct Budget<float> {
-    static constexpr int MAXE = 12;
-    static constexpr int SEG = 16;  // elements per 128-byte segment
-};
-template <>
-struct Budget<double> {
-    static constexpr int MAXE = 12;
-    static constexpr int SEG = 8;
-};
 
 // ------------------------------------------------------------------ sizes
 static bool smooth7(int v)
@@ -43,11 +27,16 @@ static bool smooth7(int v)
     return v == 1;
 }
 
-int choose_padded(int n, int mult)
+int choose_padded(int n, int mult, int plan_mult)
 {
-    int need = std::max(1, 2 * n - 1);
-    for (int P = std::max(need, 1);; ++P)
-        if (P % mult == 0 && smooth7(P)) return P;
+    const int need = std::max(1, 2 * n - 1);
+    int smooth = need;
+    while (!(smooth % mult == 0 && smooth7(smooth))) ++smooth;
+    // prefer a size with a compile-time plan unless it pads more than 12 %
+    int planned = need;
+    while (planned <= 1024 && !(has_plan(planned) && planned % plan_mult == 0)) ++planned;
+    if (planned <= 1024 && planned <= smooth + smooth / 8) return planned;
+    return smooth;
 }
 
 AxisPlan make_axis_plan(int P, bool last8)
@@ -139,265 +128,6 @@ __global__ void k_green_eval(c64* __restrict__ out, MbtDims d, int nc, double lf
     }
 }
 
-// Batched FFT along the middle axis of in[outer][nlin][Zdim] into
-// out[outer][nlout][Zdim] (zero padding nlin -> P, keeping nlout outputs).
-// A group owns Tz consecutive elements of the contiguous Zdim axis.
-template <typename R, bool INV>
-__global__ __launch_bounds__(NT_PASS) void k_lines_strided(const cplx<R>* __restrict__ in,
-                                                          cplx<R>* __restrict__ out, int nlin,
-                                                          int nlout, int64_t Zdim, int64_t ntiles,
-                                                          AxisPlan ap, const cplx<R>* __restrict__ twg,
-                                                          int Tz)
-{
-    extern __shared__ unsigned char smem_raw[];
-    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
-    cplx<R>* lds = tw + ap.P;
-    const int P = ap.P;
-    const int64_t outer = blockIdx.x / ntiles;
-    const int64_t z0 = (blockIdx.x % ntiles) * (int64_t)Tz;
-    const int tz = (int)min((int64_t)Tz, Zdim - z0);
-    for (int i = threadIdx.x; i < P; i += NT_PASS) tw[i] = twg[i];
-    const cplx<R>* src = in + (outer * nlin) * Zdim + z0;
-    for (int e = threadIdx.x; e < P * Tz; e += NT_PASS) {
-        const int p = e / Tz, t = e % Tz;
-        cplx<R> v = czero<R>();
-        if (p < nlin && t < tz) v = src[(int64_t)p * Zdim + t];
-        lds[p * Tz + t] = v;
-    }
-    __syncthreads();
-    const StageGeom g{P, Tz, Tz, 1, true};
-    run_stages<Budget<R>::MAXE, NT_PASS, R, INV>(lds, tw, g, ap, 0, ap.nst);
-    cplx<R>* dst = out + (outer * nlout) * Zdim + z0;
-    for (int e = threadIdx.x; e < nlout * Tz; e += NT_PASS) {
-        const int p = e / Tz, t = e % Tz;
-        if (t < tz) dst[(int64_t)p * Zdim + t] = lds[p * Tz + t];
-    }
-}
-
-// z pass forward with the scatter of the occupied vector fused in:
-// U[((rhs nx + ix) ny + iy) Pz + kz][c] = FFT_z( X[f c2o[cell] + c] ).
-// A group owns TL consecutive (ix, iy) lines with all F components.
-template <typename R, int F>
-__global__ __launch_bounds__(NT_PASS) void k_zfwd_scatter(const cplx<R>* __restrict__ X, int64_t ldx,
-                                                         const int* __restrict__ c2o,
-                                                         cplx<R>* __restrict__ U, MbtDims d,
-                                                         AxisPlan ap, const cplx<R>* __restrict__ twg,
-                                                         int TL, int64_t ntiles)
-{
-    extern __shared__ unsigned char smem_raw[];
-    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
-    cplx<R>* lds = tw + ap.P;
-    const int Pz = ap.P;
-    const int Tp = TL * F;
-    const int64_t nxny = (int64_t)d.nx * d.ny;
-    const int rhs = (int)(blockIdx.x / ntiles);
-    const int64_t l0 = (blockIdx.x % ntiles) * (int64_t)TL;
-    for (int i = threadIdx.x; i < Pz; i += NT_PASS) tw[i] = twg[i];
-    const cplx<R>* Xr = X + rhs * ldx;
-    for (int e = threadIdx.x; e < TL * Pz * F; e += NT_PASS) {
-        const int c = e % F;
-        const int rest = e / F;
-        const int p = rest % Pz, tl = rest / Pz;
-        const int64_t l = l0 + tl;
-        cplx<R> v = czero<R>();
-        if (l < nxny && p < d.nz) {
-            const int64_t cell = l * d.nz + p;
-            const int64_t m = c2o ? (int64_t)c2o[cell] : cell;
-            if (m >= 0) v = Xr[m * F + c];
-        }
-        lds[p * Tp + tl * F + c] = v;
-    }
-    __syncthreads();
-    const StageGeom g{Pz, Tp, Tp, 1, true};
-    run_stages<Budget<R>::MAXE, NT_PASS, R, false>(lds, tw, g, ap, 0, ap.nst);
-    cplx<R>* dst = U + ((int64_t)rhs * nxny + l0) * Pz * F;
-    const int64_t nl = min((int64_t)TL, nxny - l0);
-    for (int e = threadIdx.x; e < nl * Pz * F; e += NT_PASS) {
-        const int c = e % F;
-        const int rest = e / F;
-        const int p = rest % Pz, tl = rest / Pz;
-        dst[e] = lds[p * Tp + tl * F + c];
-    }
-}
-
-// z pass inverse with the gather fused in:
-// Y[f m + c] = diag[f m + c] X[f m + c] + IFFT_z(U)[cell][c]   (then * jpre)
-template <typename R, int F>
-__global__ __launch_bounds__(NT_PASS) void k_zinv_gather(const cplx<R>* __restrict__ U,
-                                                        const int* __restrict__ c2o,
-                                                        const cplx<R>* __restrict__ X, int64_t ldx,
-                                                        const cplx<R>* __restrict__ diag,
-                                                        const cplx<R>* __restrict__ jpre,
-                                                        cplx<R>* __restrict__ Y, int64_t ldy, MbtDims d,
-                                                        AxisPlan ap, const cplx<R>* __restrict__ twg,
-                                                        int TL, int64_t ntiles)
-{
-    extern __shared__ unsigned char smem_raw[];
-    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
-    cplx<R>* lds = tw + ap.P;
-    const int Pz = ap.P;
-    const int Tp = TL * F;
-    const int64_t nxny = (int64_t)d.nx * d.ny;
-    const int rhs = (int)(blockIdx.x / ntiles);
-    const int64_t l0 = (blockIdx.x % ntiles) * (int64_t)TL;
-    for (int i = threadIdx.x; i < Pz; i += NT_PASS) tw[i] = twg[i];
-    const cplx<R>* src = U + ((int64_t)rhs * nxny + l0) * Pz * F;
-    const int64_t nl = min((int64_t)TL, nxny - l0);
-    for (int e = threadIdx.x; e < TL * Pz * F; e += NT_PASS) {
-        const int c = e % F;
-        const int rest = e / F;
-        const int p = rest % Pz, tl = rest / Pz;
-        cplx<R> v = czero<R>();
-        if (tl < nl) v = src[e];
-        lds[p * Tp + tl * F + c] = v;
-    }
-    __syncthreads();
-    const StageGeom g{Pz, Tp, Tp, 1, true};
-    run_stages<Budget<R>::MAXE, NT_PASS, R, true>(lds, tw, g, ap, 0, ap.nst);
-    const cplx<R>* Xr = X + rhs * ldx;
-    cplx<R>* Yr = Y + rhs * ldy;
-    for (int e = threadIdx.x; e < nl * d.nz * F; e += NT_PASS) {
-        const int c = e % F;
-        const int rest = e / F;
-        const int p = rest % d.nz, tl = rest / d.nz;
-        const int64_t cell = (l0 + tl) * d.nz + p;
-        const int64_t m = c2o ? (int64_t)c2o[cell] : cell;
-        if (m < 0) continue;
-        const int64_t i = m * F + c;
-        cplx<R> v = lds[p * Tp + tl * F + c];
-        if (diag) {
-            cplx<R> dv = diag[i] * Xr[i];
-            v = dv + v;
-        }
-        if (jpre) v = v * jpre[i];
-        Yr[i] = v;
-    }
-}
-
-// Sign of a reduced G^ component when frequency axes are reflected.
-// flips: bit0 = x, bit1 = y, bit2 = z reflected.  Odd-axis masks:
-//   M00: xx,yy,zz even; xy odd in x,y; xz odd in x,z; yz odd in y,z
-//   M01 (antisymmetric): xy ~ u_z odd in z; xz ~ u_y odd in y; yz ~ u_x odd in x
-__device__ __forceinline__ int comp_odd_mask(int c)
-{
-    constexpr int masks[9] = {0, 0, 0, 3, 5, 6, 4, 2, 1};
-    return masks[c];
-}
-
-// x pass: forward FFT along x, per-frequency f x f multiply by G^, inverse
-// FFT along x, in place on V.  The forward's last stage (radix 8) keeps its
-// outputs in registers, the multiply runs there, and the inverse's first
-// stage (radix 8, no twiddles) starts from the same registers.
-template <typename R, int F>
-__global__ __launch_bounds__(NT_PASS) void k_xfused(cplx<R>* __restrict__ V, const cplx<R>* __restrict__ Gh,
-                                                   MbtDims d, AxisPlan ap, const cplx<R>* __restrict__ twg,
-                                                   int Tq, int64_t ntiles)
-{
-    constexpr int RL = 8;
-    constexpr int NC = (F == 3) ? 6 : 9;
-    extern __shared__ unsigned char smem_raw[];
-    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
-    cplx<R>* lds = tw + ap.P;
-    const int Px = ap.P;
-    const int64_t PyPz = (int64_t)d.Py * d.Pz;
-    const int Tp = Tq * F;
-    const int rhs = (int)(blockIdx.x / ntiles);
-    const int64_t q0 = (blockIdx.x % ntiles) * (int64_t)Tq;
-    const int tqn = (int)min((int64_t)Tq, PyPz - q0);
-    for (int i = threadIdx.x; i < Px; i += NT_PASS) tw[i] = twg[i];
-    cplx<R>* base = V + ((int64_t)rhs * d.nx * PyPz + q0) * F;
-    for (int e = threadIdx.x; e < Px * Tp; e += NT_PASS) {
-        const int p = e / Tp, t = e % Tp;
-        cplx<R> v = czero<R>();
-        if (p < d.nx && t < tqn * F) v = base[(int64_t)p * PyPz * F + t];
-        lds[p * Tp + t] = v;
-    }
-    __syncthreads();
-    const StageGeom g{Px, Tp, Tp, 1, true};
-    run_stages<Budget<R>::MAXE, NT_PASS, R, false>(lds, tw, g, ap, 0, ap.nst - 1);
-
-    const int nb = Px / RL;  // butterflies per line in the last stage; L = nb
-    const int total = nb * Tq;
-    const int b = threadIdx.x;  // one frequency-line butterfly per thread (host checks total <= NT)
-    cplx<R> h[F][RL];
-    int tq = 0, j = 0;
-    if (b < total) {
-        tq = b % Tq;
-        j = b / Tq;
-#pragma unroll
-        for (int c = 0; c < F; ++c)
-#pragma unroll
-            for (int r = 0; r < RL; ++r) {
-                cplx<R> x = lds[(j + r * nb) * Tp + tq * F + c];
-                if (r > 0 && j > 0) x = x * tw[r * j];
-                h[c][r] = x;
-            }
-    }
-    __syncthreads();
-    if (b < total) {
-#pragma unroll
-        for (int c = 0; c < F; ++c) Dft<RL, R, false>::run(h[c]);
-        if (tq < tqn) {
-            const int64_t q = q0 + tq;
-            const int ky = (int)(q / d.Pz), kz = (int)(q % d.Pz);
-            const int Hy = d.Py / 2 + 1, Hz = d.Pz / 2 + 1;
-            int fl = 0;
-            int kyr = ky, kzr = kz;
-            if (ky > d.Py / 2) { kyr = d.Py - ky; fl |= 2; }
-            if (kz > d.Pz / 2) { kzr = d.Pz - kz; fl |= 4; }
-#pragma unroll
-            for (int r = 0; r < RL; ++r) {
-                const int kx = j + r * nb;
-                int flr = fl, kxr = kx;
-                if (kx > Px / 2) { kxr = Px - kx; flr |= 1; }
-                const cplx<R>* gp = Gh + (((int64_t)kxr * Hy + kyr) * Hz + kzr) * NC;
-                cplx<R> gv[NC];
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    cplx<R> v = gp[c];
-                    if (__popc(comp_odd_mask(c) & flr) & 1) v = -v;
-                    gv[c] = v;
-                }
-                // symmetric S = [[xx,xy,xz],[xy,yy,yz],[xz,yz,zz]]
-                const cplx<R> ex = h[0][r], ey = h[1][r], ez = h[2][r];
-                cplx<R> ox = gv[0] * ex + gv[3] * ey + gv[4] * ez;
-                cplx<R> oy = gv[3] * ex + gv[1] * ey + gv[5] * ez;
-                cplx<R> oz = gv[4] * ex + gv[5] * ey + gv[2] * ez;
-                if constexpr (F == 6) {
-                    // A = [[0,a,b],[-a,0,c],[-b,-c,0]] with a=xy, b=xz, c=yz of M01
-                    const cplx<R> hx = h[3][r], hy = h[4][r], hz = h[5][r];
-                    const cplx<R> a = gv[6], bb = gv[7], cc = gv[8];
-                    ox = ox + a * hy + bb * hz;
-                    oy = oy - a * hx + cc * hz;
-                    oz = oz - bb * hx - cc * hy;
-                    // H' = -A E + S H
-                    const cplx<R> px = gv[0] * hx + gv[3] * hy + gv[4] * hz - (a * ey + bb * ez);
-                    const cplx<R> py = gv[3] * hx + gv[1] * hy + gv[5] * hz - (cc * ez - a * ex);
-                    const cplx<R> pz = gv[4] * hx + gv[5] * hy + gv[2] * hz - (-bb * ex - cc * ey);
-                    h[3][r] = px;
-                    h[4][r] = py;
-                    h[5][r] = pz;
-                }
-                h[0][r] = ox;
-                h[1][r] = oy;
-                h[2][r] = oz;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < F; ++c) {
-            Dft<RL, R, true>::run(h[c]);
-#pragma unroll
-            for (int r = 0; r < RL; ++r) lds[(j * RL + r) * Tp + tq * F + c] = h[c][r];
-        }
-    }
-    __syncthreads();
-    run_stages<Budget<R>::MAXE, NT_PASS, R, true>(lds, tw, g, ap, 1, ap.nst);
-    for (int e = threadIdx.x; e < d.nx * Tp; e += NT_PASS) {
-        const int p = e / Tp, t = e % Tp;
-        if (t < tqn * F) base[(int64_t)p * PyPz * F + t] = lds[p * Tp + t];
-    }
-}
-
 // extract the reduced octant of the transformed kernel (and an FP32 copy)
 __global__ void k_reduce_gh(const c64* __restrict__ full, c64* __restrict__ gh64, c32* __restrict__ gh32,
                             MbtDims d, int nc)
@@ -429,20 +159,7 @@ __global__ void k_fill_int(int* p, int64_t n, int v)
 }
 
 // ------------------------------------------------------------ host side
-static int grid_for(int64_t n, int bs = 256)
-{
-    return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, bs), 1), 65536);
-}
-
-template <typename R>
-static int tile_len(int P, int64_t Zdim)
-{
-    const int cap = Budget<R>::MAXE * NT_PASS / P;
-    int t = cap;
-    if (t >= Budget<R>::SEG) t = (t / Budget<R>::SEG) * Budget<R>::SEG;
-    t = (int)std::min<int64_t>(t, Zdim);
-    return std::max(t, 1);
-}
+static int grid_for(int64_t n) { return (int)grid_cap(n); }
 
 template <typename R>
 static void upload_twiddles(int P, void** dst)
@@ -456,19 +173,6 @@ static void upload_twiddles(int P, void** dst)
     BDDA_HIP(hipMemcpy(*dst, h.data(), sizeof(cplx<R>) * P, hipMemcpyHostToDevice));
 }
 
-template <typename R, bool INV>
-static void launch_lines(const cplx<R>* in, cplx<R>* out, int nlin, int nlout, int64_t Zdim,
-                         int64_t outer, const AxisPlan& ap, const cplx<R>* tw, hipStream_t st)
-{
-    if (ap.P * 1LL > (int64_t)Budget<R>::MAXE * NT_PASS) throw HipError{hipErrorInvalidValue, "axis too long"};
-    const int Tz = tile_len<R>(ap.P, Zdim);
-    const int64_t ntiles = ceil_div(Zdim, Tz);
-    const size_t shm = sizeof(cplx<R>) * (ap.P + (size_t)ap.P * Tz);
-    hipLaunchKernelGGL((k_lines_strided<R, INV>), dim3((unsigned)(ntiles * outer)), dim3(NT_PASS), shm, st,
-                       in, out, nlin, nlout, Zdim, ntiles, ap, tw, Tz);
-    BDDA_HIP(hipGetLastError());
-}
-
 MbtOperator* mbt_create(const int n[3], int f, double lf, double kre, double kim, int want64, int want32,
                         hipStream_t stream)
 {
@@ -477,17 +181,20 @@ MbtOperator* mbt_create(const int n[3], int f, double lf, double kre, double kim
     op->d.ny = n[1];
     op->d.nz = n[2];
     op->d.f = f;
-    op->d.Px = choose_padded(n[0], 8);
-    op->d.Py = choose_padded(n[1], 1);
-    op->d.Pz = choose_padded(n[2], 1);
+    // the all-in-LDS fused x kernel (f = 6, or sizes without a plan) needs
+    // Px % 8 == 0; planned sizes serve f = 3 only
+    op->d.Px = choose_padded(n[0], 8, f == 3 ? 1 : 8);
+    op->d.Py = choose_padded(n[1], 1, 1);
+    op->d.Pz = choose_padded(n[2], 1, 1);
     op->lf = lf;
     op->kre = kre;
     op->kim = kim;
     op->nc = (f == 3) ? 6 : 9;
     op->stream = stream;
-    op->ax[0] = make_axis_plan(op->d.Px, true);
+    op->ax[0] = make_axis_plan(op->d.Px, op->d.Px % 8 == 0);
     op->ax[1] = make_axis_plan(op->d.Py, false);
     op->ax[2] = make_axis_plan(op->d.Pz, false);
+    op->Pzs = (int)ceil_div(op->d.Pz, 16) * 16;
     const MbtDims& d = op->d;
     try {
         for (int a = 0; a < 3; ++a) {
@@ -506,11 +213,13 @@ MbtOperator* mbt_create(const int n[3], int f, double lf, double kre, double kim
         BDDA_HIP(hipGetLastError());
         const c64* tw[3] = {(const c64*)op->tw64[0], (const c64*)op->tw64[1], (const c64*)op->tw64[2]};
         // z: lines along z, stride nc
-        launch_lines<double, false>(full, tmp, d.Pz, d.Pz, nc, (int64_t)d.Px * d.Py, op->ax[2], tw[2], stream);
+        launch_lines<double, false>(full, tmp, d.Pz, d.Pz, nc, nc, (int64_t)d.Px * d.Py, op->ax[2], tw[2], stream);
         // y: stride Pz nc
-        launch_lines<double, false>(tmp, full, d.Py, d.Py, (int64_t)d.Pz * nc, d.Px, op->ax[1], tw[1], stream);
+        launch_lines<double, false>(tmp, full, d.Py, d.Py, (int64_t)d.Pz * nc, (int64_t)d.Pz * nc, d.Px, op->ax[1], tw[1],
+                                    stream);
         // x: stride Py Pz nc
-        launch_lines<double, false>(full, tmp, d.Px, d.Px, (int64_t)d.Py * d.Pz * nc, 1, op->ax[0], tw[0], stream);
+        launch_lines<double, false>(full, tmp, d.Px, d.Px, (int64_t)d.Py * d.Pz * nc, (int64_t)d.Py * d.Pz * nc, 1,
+                                    op->ax[0], tw[0], stream);
         op->gh_elems = (size_t)(d.Px / 2 + 1) * (d.Py / 2 + 1) * (d.Pz / 2 + 1);
         if (want64) BDDA_HIP(hipMalloc(&op->gh64, sizeof(c64) * op->gh_elems * nc));
         if (want32) BDDA_HIP(hipMalloc(&op->gh32, sizeof(c32) * op->gh_elems * nc));
@@ -565,7 +274,7 @@ void mbt_set_address(MbtOperator* op, const int* address_host, int64_t nocc)
     op->c2o_host_addr.assign(address_host, address_host + nocc);
 }
 
-static void ensure_ws(MbtOperator* op, size_t bytes)
+void mbt_ensure_workspace(MbtOperator* op, size_t bytes)
 {
     if (op->ws_bytes >= bytes) return;
     if (op->ws) {
@@ -578,73 +287,6 @@ static void ensure_ws(MbtOperator* op, size_t bytes)
     op->ws_bytes = bytes;
 }
 
-template <typename R, int F>
-static void apply_impl(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int64_t ldy, int s,
-                       const cplx<R>* diag, const cplx<R>* jpre, const int* c2o)
-{
-    const MbtDims& d = op->d;
-    hipStream_t st = op->stream;
-    const bool is64 = sizeof(R) == 8;
-    const cplx<R>* gh = (const cplx<R>*)(is64 ? op->gh64 : op->gh32);
-    if (!gh) throw HipError{hipErrorInvalidValue, "operator built without this precision"};
-    const cplx<R>* tw[3];
-    for (int a = 0; a < 3; ++a) tw[a] = (const cplx<R>*)(is64 ? op->tw64[a] : op->tw32[a]);
-    const int64_t nxny = (int64_t)d.nx * d.ny;
-    const int64_t nU = (int64_t)s * nxny * d.Pz * F;
-    const int64_t nV = (int64_t)s * d.nx * d.Py * d.Pz * F;
-    ensure_ws(op, sizeof(cplx<R>) * (nU + nV));
-    cplx<R>* U = (cplx<R>*)op->ws;
-    cplx<R>* V = U + nU;
-    const bool tm = op->timing != 0;
-    if (tm) BDDA_HIP(hipEventRecord(op->ev[0], st));
-    // z forward + scatter
-    {
-        const int P = d.Pz;
-        if ((int64_t)P * F > (int64_t)Budget<R>::MAXE * NT_PASS) throw HipError{hipErrorInvalidValue, "Pz too long"};
-        const int TL = std::max(1, Budget<R>::MAXE * NT_PASS / (P * F));
-        const int64_t ntiles = ceil_div(nxny, TL);
-        const size_t shm = sizeof(cplx<R>) * (P + (size_t)P * TL * F);
-        hipLaunchKernelGGL((k_zfwd_scatter<R, F>), dim3((unsigned)(ntiles * s)), dim3(NT_PASS), shm, st, X, ldx, c2o,
-                           U, d, op->ax[2], tw[2], TL, ntiles);
-        BDDA_HIP(hipGetLastError());
-    }
-    if (tm) BDDA_HIP(hipEventRecord(op->ev[1], st));
-    // y forward: U[outer = rhs nx + ix][ny][Pz F] -> V[outer][Py][Pz F]
-    launch_lines<R, false>(U, V, d.ny, d.Py, (int64_t)d.Pz * F, (int64_t)s * d.nx, op->ax[1], tw[1], st);
-    if (tm) BDDA_HIP(hipEventRecord(op->ev[2], st));
-    // x fused
-    {
-        const int P = d.Px;
-        const int Tq = Budget<R>::MAXE * NT_PASS / (P * F);
-        if (Tq < 1 || (P / 8) * Tq > NT_PASS) throw HipError{hipErrorInvalidValue, "Px too long"};
-        const int64_t PyPz = (int64_t)d.Py * d.Pz;
-        const int64_t ntiles = ceil_div(PyPz, Tq);
-        const size_t shm = sizeof(cplx<R>) * (P + (size_t)P * Tq * F);
-        hipLaunchKernelGGL((k_xfused<R, F>), dim3((unsigned)(ntiles * s)), dim3(NT_PASS), shm, st, V, gh, d,
-                           op->ax[0], tw[0], Tq, ntiles);
-        BDDA_HIP(hipGetLastError());
-    }
-    if (tm) BDDA_HIP(hipEventRecord(op->ev[3], st));
-    // y inverse: V -> U keeping ny
-    launch_lines<R, true>(V, U, d.Py, d.ny, (int64_t)d.Pz * F, (int64_t)s * d.nx, op->ax[1], tw[1], st);
-    if (tm) BDDA_HIP(hipEventRecord(op->ev[4], st));
-    // z inverse + gather
-    {
-        const int P = d.Pz;
-        const int TL = std::max(1, Budget<R>::MAXE * NT_PASS / (P * F));
-        const int64_t ntiles = ceil_div(nxny, TL);
-        const size_t shm = sizeof(cplx<R>) * (P + (size_t)P * TL * F);
-        hipLaunchKernelGGL((k_zinv_gather<R, F>), dim3((unsigned)(ntiles * s)), dim3(NT_PASS), shm, st, U, c2o, X,
-                           ldx, diag, jpre, Y, ldy, d, op->ax[2], tw[2], TL, ntiles);
-        BDDA_HIP(hipGetLastError());
-    }
-    if (tm) {
-        BDDA_HIP(hipEventRecord(op->ev[5], st));
-        BDDA_HIP(hipEventSynchronize(op->ev[5]));
-        for (int i = 0; i < 5; ++i) BDDA_HIP(hipEventElapsedTime(&op->pass_ms[i], op->ev[i], op->ev[i + 1]));
-    }
-}
-
 template <typename R>
 void mbt_apply_occupied(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int64_t ldy, int s,
                         const cplx<R>* diag, const cplx<R>* jpre, int64_t nocc)
@@ -652,8 +294,7 @@ void mbt_apply_occupied(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>*
     // nocc < 0 selects the cuboid layout (MBT_fft_mvp: every cell present)
     const int* c2o = (nocc >= 0) ? op->c2o : nullptr;
     if (nocc >= 0 && !op->c2o) throw HipError{hipErrorInvalidValue, "address not set"};
-    if (op->d.f == 3) apply_impl<R, 3>(op, X, ldx, Y, ldy, s, diag, jpre, c2o);
-    else apply_impl<R, 6>(op, X, ldx, Y, ldy, s, diag, jpre, c2o);
+    mbt_apply_precision<R>(op, X, ldx, Y, ldy, s, diag, jpre, c2o);
 }
 
 template void mbt_apply_occupied<float>(MbtOperator*, const c32*, int64_t, c32*, int64_t, int, const c32*,

@@ -7,10 +7,13 @@
 // 2/3/5/7-smooth) with a per-frequency f x f multiply by the transformed
 // interaction kernel G^ (symmetry-reduced to one octant).
 //
-// Device layouts (complex T interleaved, component index innermost):
+// Device layouts (complex T interleaved):
 //   occupied vectors   X[rhs * ld + f m + c]                (reference layout)
-//   U  (after z pass)  U[((rhs nx + ix) ny + iy) Pz + kz][c]
-//   V  (after y pass)  V[((rhs nx + ix) Py + ky) Pz + kz][c]
+//   U  (after z pass)  U[(((rhs f + c) nx + ix) ny + iy) Pzs + kz]
+//   V  (after y pass)  V[(((rhs f + c) nx + ix) Py + ky) Pzs + kz]
+// One plane per component; rows are padded to Pzs (a multiple of 16
+// elements, 128 or 256 bytes) so that every tile a pass moves starts on a
+// cache-line boundary.
 //   G^ reduced         Gh[((kx' (Py/2+1) + ky') (Pz/2+1) + kz')][nc]
 //                      nc = 6 (f=3: xx yy zz xy xz yz of -G)
 //                      nc = 9 (f=6: the same + xy xz yz of -F)
@@ -26,9 +29,10 @@ struct MbtDims {
     int Px, Py, Pz;
 };
 
-// choose the padded FFT size for one axis: smallest 2/3/5/7-smooth
-// P >= 2n-1, a multiple of `mult`
-int choose_padded(int n, int mult);
+// choose the padded FFT size for one axis: the smallest size >= 2n-1 with a
+// compile-time plan (a multiple of plan_mult) unless that pads more than
+// 12 % beyond the smallest 2/3/5/7-smooth multiple of `mult`
+int choose_padded(int n, int mult, int plan_mult);
 AxisPlan make_axis_plan(int P, bool last8);
 
 enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1 };
@@ -38,6 +42,7 @@ struct MbtOperator {
     double lf;
     double kre, kim;
     AxisPlan ax[3];             // x plan ends with radix 8
+    int Pzs = 0;                // row pitch of U and V: Pz rounded up to 16
     void* tw64[3] = {nullptr, nullptr, nullptr};  // per-axis twiddle tables
     void* tw32[3] = {nullptr, nullptr, nullptr};
     void* gh64 = nullptr;       // reduced G^, FP64 (scaled by 1/(PxPyPz))
@@ -73,6 +78,20 @@ void mbt_set_address(MbtOperator* op, const int* address_host, int64_t nocc);
 template <typename R>
 void mbt_apply_occupied(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int64_t ldy,
                         int s, const cplx<R>* diag, const cplx<R>* jpre, int64_t nocc);
+
+// grow the pass workspace to at least `bytes`
+void mbt_ensure_workspace(MbtOperator* op, size_t bytes);
+
+// one precision's pass sequence (mbt_f32.hip / mbt_f64.hip); c2o == null
+// selects the cuboid layout
+template <typename R>
+void mbt_apply_precision(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int64_t ldy, int s,
+                         const cplx<R>* diag, const cplx<R>* jpre, const int* c2o);
+
+extern template void mbt_apply_precision<float>(MbtOperator*, const cplx<float>*, int64_t, cplx<float>*, int64_t,
+                                                int, const cplx<float>*, const cplx<float>*, const int*);
+extern template void mbt_apply_precision<double>(MbtOperator*, const cplx<double>*, int64_t, cplx<double>*, int64_t,
+                                                 int, const cplx<double>*, const cplx<double>*, const int*);
 
 // bytes per block matvec under the minimal fused pipeline (the algorithmic
 // model used for roofline numbers)

@@ -1,0 +1,321 @@
+// mbt_plans.hpp -- pass kernels on the compile-time-planned FFT engine and
+// the table of padded sizes that have them.
+//
+// Every padded size P in BDDA_PLAN_TABLE has a fixed radix sequence; the
+// four passes of an MBT block matvec (z forward + scatter, y forward/inverse,
+// fused x, z inverse + gather) are instantiated for it in each precision.
+// Sizes outside the table run on the all-in-LDS kernels (mbt_passes.hpp).
+#pragma once
+#include "fft_plan.hpp"
+#include "mbt.hpp"
+
+namespace bdda {
+
+// P, then its radices (forward order)
+#define BDDA_PLAN_TABLE(X) \
+    X(32, 8, 4)            \
+    X(48, 8, 6)            \
+    X(64, 8, 8)            \
+    X(96, 8, 4, 3)         \
+    X(128, 8, 4, 4)        \
+    X(160, 8, 5, 4)        \
+    X(192, 8, 6, 4)        \
+    X(224, 8, 7, 4)        \
+    X(256, 8, 8, 4)        \
+    X(280, 8, 7, 5)        \
+    X(320, 8, 8, 5)        \
+    X(343, 7, 7, 7)        \
+    X(384, 8, 8, 6)        \
+    X(448, 8, 8, 7)        \
+    X(512, 8, 8, 8)
+
+// calls f(Radices<...>{}) for the plan of size P; false when P has none
+template <class F>
+inline bool with_plan(int P, F&& f)
+{
+    switch (P) {
+#define BDDA_PLAN_CASE(p, ...) \
+    case p: f(Radices<__VA_ARGS__>{}); return true;
+        BDDA_PLAN_TABLE(BDDA_PLAN_CASE)
+#undef BDDA_PLAN_CASE
+    default: return false;
+    }
+}
+
+inline bool has_plan(int P)
+{
+    return with_plan(P, [](auto) {});
+}
+
+// tile widths of the planned passes: 128-byte row segments in every pass
+template <typename R>
+struct PlanTiles {
+    static constexpr int NT = 256;
+    static constexpr int NT_X = 512;
+    static constexpr int NQ_Y = sizeof(R) == 4 ? 16 : 8;  // y pass: columns per group
+    static constexpr int NQ_X = sizeof(R) == 4 ? 16 : 8;  // x pass: kz per group (all F planes)
+    static constexpr int TL_Z = sizeof(R) == 4 ? 8 : 4;   // z pass: lines per group
+};
+
+// ---------------------------------------------------------------- y pass
+// in[outer][nlin][pitch] -> out[outer][nlout][pitch] over the first Zdim
+// columns; a group owns NQ consecutive columns.
+template <typename R, class PL, bool INV, int NT, int NQ>
+__global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
+                                                   int nlin, int nlout, int64_t Zdim, int64_t pitch, int64_t ntiles,
+                                                   const cplx<R>* __restrict__ twg)
+{
+    constexpr int P = PL::P;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* lds = tw + P;
+    const int64_t outer = blockIdx.x / ntiles;
+    const int64_t z0 = (blockIdx.x % ntiles) * (int64_t)NQ;
+    const int tz = (int)min((int64_t)NQ, Zdim - z0);
+    for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
+    const cplx<R>* src = in + (outer * nlin) * pitch + z0;
+    cplx<R>* dst = out + (outer * nlout) * pitch + z0;
+    PlanFFT<R, PL, NT, NQ, 1, 1> F(tw, lds);
+    F.template load<INV>([&](int p, int q, int) {
+        return (p < nlin && q < tz) ? src[(int64_t)p * pitch + q] : czero<R>();
+    });
+    __syncthreads();
+    F.template run<INV>();
+    F.template store<INV>([&](int p, int q, int, cplx<R> v) {
+        if (p < nlout && q < tz) dst[(int64_t)p * pitch + q] = v;
+    });
+}
+
+// ---------------------------------------------------------------- x pass
+// A group owns (rhs, ky, NQ consecutive kz) across all F component planes.
+template <typename R, class PL, int F, int NT, int NQ>
+__global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, const cplx<R>* __restrict__ Gh,
+                                                    MbtDims d, int Pzs, const cplx<R>* __restrict__ twg, int ntz)
+{
+    constexpr int P = PL::P;
+    constexpr int NC = (F == 3) ? 6 : 9;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* lds = tw + P;
+    const int64_t plane = (int64_t)d.Py * Pzs;
+    const int64_t cstride = (int64_t)d.nx * plane;
+    const int64_t per_rhs = (int64_t)d.Py * ntz;
+    const int rhs = (int)(blockIdx.x / per_rhs);
+    const int rem = (int)(blockIdx.x % per_rhs);
+    const int ky = rem / ntz;
+    const int kz0 = (rem % ntz) * NQ;
+    const int tqn = min(NQ, d.Pz - kz0);
+    for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
+    cplx<R>* base = V + (int64_t)rhs * F * cstride + (int64_t)ky * Pzs + kz0;
+    PlanFFT<R, PL, NT, NQ, F, 1> X(tw, lds);
+    X.template load<false>([&](int p, int q, int c) {
+        return (p < d.nx && q < tqn) ? base[c * cstride + p * plane + q] : czero<R>();
+    });
+    __syncthreads();
+    X.template run<false>();
+    const int Hy = d.Py / 2 + 1, Hz = d.Pz / 2 + 1;
+    int flyz = 0, kyr = ky;
+    if (ky > d.Py / 2) { kyr = d.Py - ky; flyz |= 2; }
+    X.visit([&](int kx, int q, cplx<R>* h) {
+        if (q >= tqn) return;
+        const int kz = kz0 + q;
+        int fl = flyz, kzr = kz, kxr = kx;
+        if (kz > d.Pz / 2) { kzr = d.Pz - kz; fl |= 4; }
+        if (kx > P / 2) { kxr = P - kx; fl |= 1; }
+        const cplx<R>* gp = Gh + (((int64_t)kxr * Hy + kyr) * Hz + kzr) * NC;
+        cplx<R> gv[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            cplx<R> v = gp[c];
+            if (__popc(comp_odd_mask(c) & fl) & 1) v = -v;
+            gv[c] = v;
+        }
+        const cplx<R> ex = h[0], ey = h[1], ez = h[2];
+        cplx<R> ox = gv[0] * ex + gv[3] * ey + gv[4] * ez;
+        cplx<R> oy = gv[3] * ex + gv[1] * ey + gv[5] * ez;
+        cplx<R> oz = gv[4] * ex + gv[5] * ey + gv[2] * ez;
+        if constexpr (F == 6) {
+            const cplx<R> hx = h[3], hy = h[4], hz = h[5];
+            const cplx<R> a = gv[6], bb = gv[7], cc = gv[8];
+            ox = ox + a * hy + bb * hz;
+            oy = oy - a * hx + cc * hz;
+            oz = oz - bb * hx - cc * hy;
+            h[3] = gv[0] * hx + gv[3] * hy + gv[4] * hz - (a * ey + bb * ez);
+            h[4] = gv[3] * hx + gv[1] * hy + gv[5] * hz - (cc * ez - a * ex);
+            h[5] = gv[4] * hx + gv[5] * hy + gv[2] * hz - (-bb * ex - cc * ey);
+        }
+        h[0] = ox;
+        h[1] = oy;
+        h[2] = oz;
+    });
+    X.template run<true>();
+    X.template store<true>([&](int p, int q, int c, cplx<R> v) {
+        if (p < d.nx && q < tqn) base[c * cstride + p * plane + q] = v;
+    });
+}
+
+// ---------------------------------------------------------------- z passes
+// A group owns TL consecutive (ix, iy) lines with all F components.  Each
+// component's lines are contiguous in U, so they move between HBM and LDS in
+// coalesced sweeps; the FFT reads/writes them from LDS.
+
+// forward + scatter: U[((rhs F + c) nxny + l) Pzs + kz] = FFT_z(X[f c2o[cell] + c])
+template <typename R, class PL, int F, int NT, int TL>
+__global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X, int64_t ldx,
+                                                  const int* __restrict__ c2o, cplx<R>* __restrict__ U, MbtDims d,
+                                                  int Pzs, const cplx<R>* __restrict__ twg, int64_t ntiles)
+{
+    constexpr int P = PL::P;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* lds = tw + P;
+    const int64_t nxny = (int64_t)d.nx * d.ny;
+    const int rhs = (int)(blockIdx.x / ntiles);
+    const int64_t l0 = (blockIdx.x % ntiles) * (int64_t)TL;
+    const int nl = (int)min((int64_t)TL, nxny - l0);
+    for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
+    const cplx<R>* Xr = X + rhs * ldx;
+    // coalesced gather of the occupied values into LDS [(p TL + line) F + c]
+    for (int e = threadIdx.x; e < TL * d.nz * F; e += NT) {
+        const int c = e % F;
+        const int rest = e / F;
+        const int p = rest % d.nz, tl = rest / d.nz;
+        cplx<R> v = czero<R>();
+        if (tl < nl) {
+            const int64_t cell = (l0 + tl) * d.nz + p;
+            const int64_t m = c2o ? (int64_t)c2o[cell] : cell;
+            if (m >= 0) v = Xr[m * F + c];
+        }
+        lds[(p * TL + tl) * F + c] = v;
+    }
+    __syncthreads();
+    PlanFFT<R, PL, NT, TL, F, F> Z(tw, lds);
+    Z.template load<false>([&](int p, int q, int c) { return p < d.nz ? lds[(p * TL + q) * F + c] : czero<R>(); });
+    __syncthreads();
+    Z.template run<false>();
+    Z.template store<false>([&](int p, int q, int c, cplx<R> v) { lds[(c * TL + q) * P + p] = v; });
+    __syncthreads();
+    for (int e = threadIdx.x; e < F * nl * P; e += NT) {
+        const int p = e % P;
+        const int rest = e / P;
+        const int tl = rest % nl, c = rest / nl;
+        U[((((int64_t)rhs * F + c) * nxny) + l0 + tl) * Pzs + p] = lds[(c * TL + tl) * P + p];
+    }
+}
+
+// inverse + gather: Y[f m + c] = diag .* X + IFFT_z(U)[cell][c]  (then * jpre)
+template <typename R, class PL, int F, int NT, int TL>
+__global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U, const int* __restrict__ c2o,
+                                                  const cplx<R>* __restrict__ X, int64_t ldx,
+                                                  const cplx<R>* __restrict__ diag, const cplx<R>* __restrict__ jpre,
+                                                  cplx<R>* __restrict__ Y, int64_t ldy, MbtDims d, int Pzs,
+                                                  const cplx<R>* __restrict__ twg, int64_t ntiles)
+{
+    constexpr int P = PL::P;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* lds = tw + P;
+    const int64_t nxny = (int64_t)d.nx * d.ny;
+    const int rhs = (int)(blockIdx.x / ntiles);
+    const int64_t l0 = (blockIdx.x % ntiles) * (int64_t)TL;
+    const int nl = (int)min((int64_t)TL, nxny - l0);
+    for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
+    for (int e = threadIdx.x; e < F * TL * P; e += NT) {
+        const int p = e % P;
+        const int rest = e / P;
+        const int tl = rest % TL, c = rest / TL;
+        cplx<R> v = czero<R>();
+        if (tl < nl) v = U[((((int64_t)rhs * F + c) * nxny) + l0 + tl) * Pzs + p];
+        lds[(c * TL + tl) * P + p] = v;
+    }
+    __syncthreads();
+    PlanFFT<R, PL, NT, TL, F, F> Z(tw, lds);
+    Z.template load<true>([&](int p, int q, int c) { return lds[(c * TL + q) * P + p]; });
+    __syncthreads();
+    Z.template run<true>();
+    Z.template store<true>([&](int p, int q, int c, cplx<R> v) {
+        if (p < d.nz) lds[(q * d.nz + p) * F + c] = v;
+    });
+    __syncthreads();
+    const cplx<R>* Xr = X + rhs * ldx;
+    cplx<R>* Yr = Y + rhs * ldy;
+    for (int e = threadIdx.x; e < nl * d.nz * F; e += NT) {
+        const int c = e % F;
+        const int rest = e / F;
+        const int p = rest % d.nz, tl = rest / d.nz;
+        const int64_t cell = (l0 + tl) * d.nz + p;
+        const int64_t m = c2o ? (int64_t)c2o[cell] : cell;
+        if (m < 0) continue;
+        const int64_t i = m * F + c;
+        cplx<R> v = lds[e];
+        if (diag) v = diag[i] * Xr[i] + v;
+        if (jpre) v = v * jpre[i];
+        Yr[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+template <typename R, bool INV>
+inline bool launch_lines_plan(const cplx<R>* in, cplx<R>* out, int nlin, int nlout, int64_t Zdim, int64_t pitch,
+                              int64_t outer, int P, const cplx<R>* tw, hipStream_t st)
+{
+    using T = PlanTiles<R>;
+    return with_plan(P, [&](auto pl) {
+        using PL = decltype(pl);
+        const int64_t ntiles = ceil_div(Zdim, T::NQ_Y);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_Y);
+        hipLaunchKernelGGL((k_lines_plan<R, PL, INV, T::NT, T::NQ_Y>), dim3((unsigned)(ntiles * outer)), dim3(T::NT),
+                           shm, st, in, out, nlin, nlout, Zdim, pitch, ntiles, tw);
+        BDDA_HIP(hipGetLastError());
+    });
+}
+
+template <typename R, int F>
+inline bool launch_xfused_plan(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, int Pzs, const cplx<R>* tw, int s,
+                               hipStream_t st)
+{
+    using T = PlanTiles<R>;
+    return with_plan(d.Px, [&](auto pl) {
+        using PL = decltype(pl);
+        const int ntz = (int)ceil_div(d.Pz, T::NQ_X);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_X);
+        hipLaunchKernelGGL((k_xfused_plan<R, PL, F, T::NT_X, T::NQ_X>), dim3((unsigned)((int64_t)s * d.Py * ntz)),
+                           dim3(T::NT_X), shm, st, V, gh, d, Pzs, tw, ntz);
+        BDDA_HIP(hipGetLastError());
+    });
+}
+
+template <typename R, int F>
+inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx<R>* U, const MbtDims& d, int Pzs,
+                             const cplx<R>* tw, int s, hipStream_t st)
+{
+    using T = PlanTiles<R>;
+    return with_plan(d.Pz, [&](auto pl) {
+        using PL = decltype(pl);
+        const int64_t nxny = (int64_t)d.nx * d.ny;
+        const int64_t ntiles = ceil_div(nxny, T::TL_Z);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::TL_Z * F);
+        hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, T::NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(T::NT), shm, st,
+                           X, ldx, c2o, U, d, Pzs, tw, ntiles);
+        BDDA_HIP(hipGetLastError());
+    });
+}
+
+template <typename R, int F>
+inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X, int64_t ldx, const cplx<R>* diag,
+                             const cplx<R>* jpre, cplx<R>* Y, int64_t ldy, const MbtDims& d, int Pzs,
+                             const cplx<R>* tw, int s, hipStream_t st)
+{
+    using T = PlanTiles<R>;
+    return with_plan(d.Pz, [&](auto pl) {
+        using PL = decltype(pl);
+        const int64_t nxny = (int64_t)d.nx * d.ny;
+        const int64_t ntiles = ceil_div(nxny, T::TL_Z);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::TL_Z * F);
+        hipLaunchKernelGGL((k_zinv_plan<R, PL, F, T::NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(T::NT), shm, st,
+                           U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles);
+        BDDA_HIP(hipGetLastError());
+    });
+}
+
+}  // namespace bdda
