@@ -16,7 +16,7 @@
 namespace bdda {
 
 constexpr int NT_RED = 256;
-constexpr int MAX_PARTIAL_BLOCKS = 1024;
+constexpr int MAX_PARTIAL_BLOCKS = 512;
 constexpr int COEF_SLOTS = 64;
 constexpr int COEF_SLOT_ELEMS = 2 * 64 * 64;
 
@@ -108,17 +108,27 @@ __global__ __launch_bounds__(NT_RED) void k_gram_partial(const cplx<R>* __restri
     }
 }
 
-__global__ void k_reduce_partial(const c64* __restrict__ partial, int nblk, int np, c64* __restrict__ out)
+// out[p] = sum_k partial[k np + p]; one 256-thread group per 64 entries:
+// four k-slices per entry, combined in LDS in a fixed order (deterministic)
+__global__ __launch_bounds__(256) void k_reduce_partial(const c64* __restrict__ partial, int nblk, int np,
+                                                        c64* __restrict__ out)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
+    __shared__ double sh[2][4][64];
+    const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
     double sx = 0, sy = 0;
-    for (int k = 0; k < nblk; ++k) {
-        const c64 v = partial[(int64_t)k * np + p];
-        sx += v.x;
-        sy += v.y;
-    }
-    out[p] = mk(sx, sy);
+    if (p < np)
+        for (int k = slice; k < nblk; k += 4) {
+            const c64 v = partial[(int64_t)k * np + p];
+            sx += v.x;
+            sy += v.y;
+        }
+    sh[0][slice][lane] = sx;
+    sh[1][slice][lane] = sy;
+    __syncthreads();
+    if (slice == 0 && p < np)
+        out[p] = mk(((sh[0][0][lane] + sh[0][1][lane]) + sh[0][2][lane]) + sh[0][3][lane],
+                    ((sh[1][0][lane] + sh[1][1][lane]) + sh[1][2][lane]) + sh[1][3][lane]);
 }
 
 template <typename R, bool CONJ>
@@ -265,7 +275,7 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
         hipLaunchKernelGGL((k_gram_partial<R, false>), dim3(nblk), dim3(NT_RED), shm, w.stream, A, lda, a, B, ldb, b,
                            rows, TR, rpb, w.partial);
     BDDA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0, w.stream,
+    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, 64)), dim3(256), 0, w.stream,
                        (const c64*)w.partial, nblk, np, w.red);
     BDDA_HIP(hipGetLastError());
     HostMat out(np);

@@ -175,18 +175,17 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
     const int nl = (int)min((int64_t)TL, nxny - l0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     const cplx<R>* Xr = X + rhs * ldx;
-    // coalesced gather of the occupied values into LDS [(p TL + line) F + c]
-    for (int e = threadIdx.x; e < TL * d.nz * F; e += NT) {
-        const int c = e % F;
-        const int rest = e / F;
-        const int p = rest % d.nz, tl = rest / d.nz;
-        cplx<R> v = czero<R>();
+    // gather of the occupied values into LDS [(p TL + line) F + c], one
+    // occupancy lookup per cell
+    for (int e = threadIdx.x; e < TL * d.nz; e += NT) {
+        const int p = e % d.nz, tl = e / d.nz;
+        int64_t m = -1;
         if (tl < nl) {
             const int64_t cell = (l0 + tl) * d.nz + p;
-            const int64_t m = c2o ? (int64_t)c2o[cell] : cell;
-            if (m >= 0) v = Xr[m * F + c];
+            m = c2o ? (int64_t)c2o[cell] : cell;
         }
-        lds[(p * TL + tl) * F + c] = v;
+#pragma unroll
+        for (int c = 0; c < F; ++c) lds[(p * TL + tl) * F + c] = (m >= 0) ? Xr[m * F + c] : czero<R>();
     }
     __syncthreads();
     PlanFFT<R, PL, NT, TL, F, F> Z(tw, lds);
@@ -239,18 +238,19 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
     __syncthreads();
     const cplx<R>* Xr = X + rhs * ldx;
     cplx<R>* Yr = Y + rhs * ldy;
-    for (int e = threadIdx.x; e < nl * d.nz * F; e += NT) {
-        const int c = e % F;
-        const int rest = e / F;
-        const int p = rest % d.nz, tl = rest / d.nz;
-        const int64_t cell = (l0 + tl) * d.nz + p;
+    // one occupancy lookup per cell; the cells of a line are consecutive
+    for (int e = threadIdx.x; e < nl * d.nz; e += NT) {
+        const int64_t cell = l0 * d.nz + e;
         const int64_t m = c2o ? (int64_t)c2o[cell] : cell;
         if (m < 0) continue;
-        const int64_t i = m * F + c;
-        cplx<R> v = lds[e];
-        if (diag) v = diag[i] * Xr[i] + v;
-        if (jpre) v = v * jpre[i];
-        Yr[i] = v;
+#pragma unroll
+        for (int c = 0; c < F; ++c) {
+            const int64_t i = m * F + c;
+            cplx<R> v = lds[e * F + c];
+            if (diag) v = diag[i] * Xr[i] + v;
+            if (jpre) v = v * jpre[i];
+            Yr[i] = v;
+        }
     }
 }
 

@@ -431,7 +431,10 @@ int solve_fp(Problem& P, int method, int jacobi, int mode, const c64* B64, c64* 
                 break;
             }
             if (o > 0 && total_iters >= itermax) break;
-            double itol = inner_tol > 0 ? inner_tol : 0.3 * tol / rel;
+            // inner FP32 solve to the outer target: the FP32 recurrence drifts
+            // from the true residual by a small factor, and a second, short
+            // correction step is cheaper than over-solving the first one
+            double itol = inner_tol > 0 ? inner_tol : 0.9 * tol / rel;
             itol = std::min(std::max(itol, 2e-6), 0.5);
             convert_64_to_32(w.stream, R64.p, rows, R32.p, rows, rows, s, 1.0 / rn);
             SolveInfo inner;
