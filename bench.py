@@ -179,7 +179,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solve path is measured on a single MI355X "
-              "(multi-GPU sharding by RHS columns is described in DESIGN.md, not run).", file=sys.stderr)
+              "(the path would shard by right-hand-side columns with an s x s all-reduce per iteration; "
+              "that is not run here).", file=sys.stderr)
         sys.exit(2)
     pkg = load_pkg()
     cfg = CONFIGS[args.config]

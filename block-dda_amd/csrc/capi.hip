@@ -165,7 +165,7 @@ static int solve_common(bdda_mbt* h, int method, int jacobi, int mode, const c64
         info->time_ms = si.time_ms;
     }
     if (hist)
-        for (int i = 0; i < hist_len; ++i) hist[i] = (i < (int)hv.size()) ? hv[i] : 0.0;
+        for (int i = 0; i < hist_len; ++i) hist[i] = (i < (int)hv.size()) ? hv[i] : std::nan("");
     if (rc == BDDA_EBREAKDOWN) set_error("block breakdown: rank-deficient block (QR or s x s LU)");
     else if (rc == BDDA_ENAN) set_error("NaN or Inf in the iteration");
     else if (rc == BDDA_ENOCONV) set_error("itermax reached before tol");

@@ -109,7 +109,8 @@ int bdda_mbt_apply_occupied(bdda_mbt* h, int prec, int jacobi, const bdda_c64* X
  * iteration in complex FP32 inside an FP64 defect-correction loop that stops
  * on the true FP64 residual ||B - A X||/||B|| <= tol.  hist (optional,
  * hist_len entries) receives per-iteration residuals (outer true residuals
- * are stored negated).  Returns BDDA_ENOCONV when itermax was reached. */
+ * are stored negated; entries past the last one written are NaN).  Returns
+ * BDDA_ENOCONV when itermax was reached. */
 int bdda_solve(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64* B, int64_t ldb, int L, double tol,
                int itermax, bdda_c64* X, int64_t ldx, bdda_solve_info* info, double* hist, int hist_len);
 
