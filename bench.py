@@ -48,29 +48,14 @@ def load_pkg():
     return mod
 
 
-CONFIGS = {
-    # name: (monomers, dipoles_across, m, wavelength_nm, directions, seed)
-    "agg16384": dict(N=16384, Df=2.8, kf=1.0, a0=20.0, across=4, m=2.3 + 0.6j, wl=400.0, ndir=8, seed=16384),
-    "agg1024": dict(N=1024, Df=2.8, kf=1.0, a0=20.0, across=8, m=1.95 + 0.79j, wl=550.0, ndir=16, seed=1024),
-    "agg256": dict(N=256, Df=2.8, kf=1.0, a0=20.0, across=4, m=2.3 + 0.6j, wl=400.0, ndir=8, seed=256),
-}
-
-
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(pkg, cfg):
-    T = importlib.import_module("block_dda_amd.targets")
-    t0 = time.time()
-    tgt = T.aggregate_target(cfg["N"], cfg["Df"], cfg["kf"], cfg["a0"], cfg["across"], cfg["seed"])
-    k = 2 * math.pi / cfg["wl"]
-    alpha = T.alpha_ldr(cfg["m"], tgt.lf, k, S=0.0)
-    diag = T.diag_from_alpha(alpha, tgt.nocc, 3)
-    waves = T.plane_waves_sphere(k, cfg["ndir"], cfg["seed"])
-    B = waves.rhs(tgt)
-    log("workload: %s grid %s nocc %d s %d (%.1fs)" % (tgt.name, tgt.n, tgt.nocc, B.shape[1], time.time() - t0))
-    return tgt, k, diag, B
+def build_workload(name):
+    W = importlib.import_module("block_dda_amd.workloads").build(name)
+    log("workload: %s (%.1fs)" % (W.describe(), W.seconds))
+    return W
 
 
 def pass_bytes(n, P, nocc, s, f, c, nc):
@@ -170,7 +155,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="agg16384", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="agg16384", choices=["agg16384", "agg1024", "agg256", "sphere16"])
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--itermax", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -183,8 +168,8 @@ def main():
               "that is not run here).", file=sys.stderr)
         sys.exit(2)
     pkg = load_pkg()
-    cfg = CONFIGS[args.config]
-    tgt, k, diag, B = build_workload(pkg, cfg)
+    W = build_workload(args.config)
+    tgt, k, diag, B = W.target, W.k, W.diag, W.B
     s = B.shape[1]
     t0 = time.time()
     op = pkg.MBT_fft_init(tgt.n, 3, tgt.lf, k)
@@ -253,10 +238,7 @@ def main():
         "metric": METRIC, "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "complex64 inner + complex128 outer", "data": "synthetic (seeded aggregate, plane waves)",
-        "config": {"workload": "%s: %d monomers a0=20nm Df=2.8 kf=1.0, %d dipoles across, grid %s -> %s, "
-                               "N=%d dipoles, m=%s, lambda=%gnm, s=%d, block COCG-rq (Jacobi), tol %g"
-                               % (args.config, cfg["N"], cfg["across"], list(tgt.n), list(P), tgt.nocc,
-                                  cfg["m"], cfg["wl"], s, args.tol),
+        "config": {"workload": "%s -> padded %s, block COCG-rq (Jacobi), tol %g" % (W.describe(), list(P), args.tol),
                    "parallelism": "single GPU"},
         "mixed": {"iters": runs[-1]["iters"], "outer": runs[-1]["outer_iters"], "matvecs": runs[-1]["matvecs"],
                   "true_res": runs[-1]["true_res"], "converged": runs[-1]["converged"]},

@@ -16,7 +16,7 @@
 namespace bdda {
 
 constexpr int NT_RED = 256;
-constexpr int MAX_PARTIAL_BLOCKS = 512;
+constexpr int MAX_PARTIAL_BLOCKS = 1024;
 constexpr int COEF_SLOTS = 64;
 constexpr int COEF_SLOT_ELEMS = 2 * 64 * 64;
 
@@ -58,54 +58,92 @@ __device__ __forceinline__ c64 cmul64(cplx<R> a, cplx<R> b)
     return mk(ax * bx - ay * by, ax * by + ay * bx);
 }
 
+// Gram partial sums: each work-group covers a contiguous range of rows and
+// writes its a x b FP64 partial to partial[blockIdx.x * a * b + ...].
+// Row tiles of TR rows are staged in LDS (columns padded with zeros to a
+// multiple of 4); thread t owns a 4 x 4 register tile of output pairs
+// (tile t % NTILE) and the rows r = t / NTILE (mod RG) of every LDS tile,
+// accumulating in FP64.  The RG row groups are summed in LDS at the end in
+// a fixed order, so the result does not depend on scheduling.
 template <typename R, bool CONJ>
 __global__ __launch_bounds__(NT_RED) void k_gram_partial(const cplx<R>* __restrict__ A, int64_t lda, int a,
                                                         const cplx<R>* __restrict__ B, int64_t ldb, int b,
                                                         int64_t rows, int TR, int64_t rows_per_blk,
                                                         c64* __restrict__ partial)
 {
-    constexpr int PPT = 16;  // pairs per thread (a*b <= 4096)
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int ta = (a + 3) >> 2, tb = (b + 3) >> 2;
+    const int las = 4 * ta, lbs = 4 * tb;  // LDS row pitch (elements)
     cplx<R>* As = reinterpret_cast<cplx<R>*>(smem_raw);
-    cplx<R>* Bs = As + (size_t)TR * a;
-    const int np = a * b;
-    c64 acc[PPT];
+    cplx<R>* Bs = As + (size_t)TR * las;
+    const int ntile = ta * tb;
+    const int RG = NT_RED / ntile;
+    const int tid = threadIdx.x;
+    const int tile = tid % ntile, rg = tid / ntile;
+    const bool active = rg < RG;
+    const int ia0 = 4 * (tile % ta), ib0 = 4 * (tile / ta);
+    // zero the padding columns once (tile loads never write them)
+    for (int e = tid; e < TR * (las - a); e += NT_RED) As[(e / (las - a)) * las + a + e % (las - a)] = czero<R>();
+    for (int e = tid; e < TR * (lbs - b); e += NT_RED) Bs[(e / (lbs - b)) * lbs + b + e % (lbs - b)] = czero<R>();
+    double acc[4][4][2];
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) acc[i] = mk(0.0, 0.0);
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q][0] = acc[p][q][1] = 0.0;
     const int64_t rb = blockIdx.x * rows_per_blk;
     const int64_t re = min(rows, rb + rows_per_blk);
     for (int64_t r0 = rb; r0 < re; r0 += TR) {
         const int nr = (int)min((int64_t)TR, re - r0);
-        for (int e = threadIdx.x; e < TR * a; e += NT_RED) {
-            const int r = e % TR, col = e / TR;
-            As[r * a + col] = (r < nr) ? A[col * lda + r0 + r] : czero<R>();
+        for (int e = tid; e < nr * a; e += NT_RED) {
+            const int r = e % nr, col = e / nr;
+            As[r * las + col] = A[col * lda + r0 + r];
         }
-        for (int e = threadIdx.x; e < TR * b; e += NT_RED) {
-            const int r = e % TR, col = e / TR;
-            Bs[r * b + col] = (r < nr) ? B[col * ldb + r0 + r] : czero<R>();
+        for (int e = tid; e < nr * b; e += NT_RED) {
+            const int r = e % nr, col = e / nr;
+            Bs[r * lbs + col] = B[col * ldb + r0 + r];
         }
         __syncthreads();
+        if (active)
+            for (int r = rg; r < nr; r += RG) {
+                cplx<R> x[4], y[4];
 #pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            const int p = threadIdx.x + i * NT_RED;
-            if (p < np) {
-                const int ia = p % a, ib = p / a;
-                c64 s = acc[i];
-                for (int r = 0; r < nr; ++r) {
-                    const c64 t = cmul64<R, CONJ>(As[r * a + ia], Bs[r * b + ib]);
-                    s.x += t.x;
-                    s.y += t.y;
+                for (int p = 0; p < 4; ++p) x[p] = As[r * las + ia0 + p];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) y[q] = Bs[r * lbs + ib0 + q];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    const double xr = x[p].x, xi = CONJ ? -(double)x[p].y : (double)x[p].y;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const double yr = y[q].x, yi = y[q].y;
+                        acc[p][q][0] = fma(xr, yr, fma(-xi, yi, acc[p][q][0]));
+                        acc[p][q][1] = fma(xr, yi, fma(xi, yr, acc[p][q][1]));
+                    }
                 }
-                acc[i] = s;
             }
-        }
         __syncthreads();
     }
+    // sum the RG row groups: one 4 x 4 entry per round through LDS
+    double* red = reinterpret_cast<double*>(smem_raw);
+    const int np = a * b;
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-        const int p = threadIdx.x + i * NT_RED;
-        if (p < np) partial[(int64_t)blockIdx.x * np + p] = acc[i];
-    }
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            red[2 * tid] = acc[p][q][0];
+            red[2 * tid + 1] = acc[p][q][1];
+            __syncthreads();
+            if (active && rg == 0) {
+                double sx = 0, sy = 0;
+                for (int g = 0; g < RG; ++g) {
+                    sx += red[2 * (g * ntile + tile)];
+                    sy += red[2 * (g * ntile + tile) + 1];
+                }
+                const int ia = ia0 + p, ib = ib0 + q;
+                if (ia < a && ib < b) partial[(int64_t)blockIdx.x * np + ia + (int64_t)ib * a] = mk(sx, sy);
+            }
+            __syncthreads();
+        }
 }
 
 // out[p] = sum_k partial[k np + p]; one 256-thread group per 64 entries:
@@ -138,13 +176,12 @@ __global__ __launch_bounds__(NT_RED) void k_dot_partial(const cplx<R>* __restric
 {
     __shared__ double sh[2][NT_RED];
     double sx = 0, sy = 0;
-    const int64_t total = rows * s;
-    for (int64_t i = blockIdx.x * (int64_t)NT_RED + threadIdx.x; i < total; i += (int64_t)gridDim.x * NT_RED) {
-        const int64_t r = i % rows, j = i / rows;
-        const c64 t = cmul64<R, CONJ>(A[j * lda + r], B[j * ldb + r]);
-        sx += t.x;
-        sy += t.y;
-    }
+    for (int j = 0; j < s; ++j)
+        for (int64_t r = blockIdx.x * (int64_t)NT_RED + threadIdx.x; r < rows; r += (int64_t)gridDim.x * NT_RED) {
+            const c64 t = cmul64<R, CONJ>(A[j * lda + r], B[j * ldb + r]);
+            sx += t.x;
+            sy += t.y;
+        }
     sh[0][threadIdx.x] = sx;
     sh[1][threadIdx.x] = sy;
     __syncthreads();
@@ -172,29 +209,54 @@ struct ComboDev {
 };
 
 // Z[:, j0 + jj] = sum_t a_t V_t[:, j0+jj] + sum_u sum_l W_u[:, l] M_u[l, j0+jj]
-// for jj < nj <= SJ.  One row per thread; the coefficient reads are
-// wave-uniform.  Z may alias a V_t (same element, same thread) but not a W_u.
-template <typename R, int SJ>
+// for jj < nj <= SJ, kw_u <= KW.  The coefficient block M_u[:, j0:j0+SJ]
+// sits in LDS (zero-padded to KW x SJ) and is read by every lane at the same
+// address; each thread owns one row and issues all of its loads before its
+// multiply-adds.  Z may alias a V_t (same element, same thread) but not a
+// W_u.
+template <typename R, int KW, int SJ>
 __global__ __launch_bounds__(256) void k_combo(ComboDev<R> c, cplx<R>* Z, int64_t ldz, int64_t rows, int j0, int nj)
 {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+    __shared__ cplx<R> Ms[2][KW > 0 ? KW : 1][SJ];
+    if constexpr (KW > 0) {
+        for (int u = 0; u < c.nm; ++u)
+            for (int e = threadIdx.x; e < KW * SJ; e += blockDim.x) {
+                const int l = e / SJ, jj = e % SJ;
+                Ms[u][l][jj] = (l < c.kw[u] && jj < nj) ? c.M[u][l + (int64_t)(j0 + jj) * c.kw[u]] : czero<R>();
+            }
+        __syncthreads();
+    }
+    // one row per thread: a grid-stride loop would let the compiler hoist
+    // the whole LDS coefficient block into registers
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r < rows) {
         cplx<R> acc[SJ];
 #pragma unroll
-        for (int jj = 0; jj < SJ; ++jj) {
-            acc[jj] = czero<R>();
-            if (jj < nj)
-                for (int t = 0; t < c.nv; ++t) fma_c(acc[jj], c.a[t], c.V[t][(int64_t)(j0 + jj) * c.ldv[t] + r]);
-        }
-        for (int u = 0; u < c.nm; ++u) {
-            const int kw = c.kw[u];
-            const cplx<R>* W = c.W[u];
-            const cplx<R>* M = c.M[u];
-            for (int l = 0; l < kw; ++l) {
-                const cplx<R> w = W[(int64_t)l * c.ldw[u] + r];
+        for (int jj = 0; jj < SJ; ++jj) acc[jj] = czero<R>();
 #pragma unroll
-                for (int jj = 0; jj < SJ; ++jj)
-                    if (jj < nj) fma_c(acc[jj], w, M[l + (int64_t)(j0 + jj) * kw]);
+        for (int t = 0; t < 3; ++t)
+            if (t < c.nv) {
+                const cplx<R>* V = c.V[t] + (int64_t)j0 * c.ldv[t] + r;
+                cplx<R> v[SJ];
+#pragma unroll
+                for (int jj = 0; jj < SJ; ++jj) v[jj] = (jj < nj) ? V[(int64_t)jj * c.ldv[t]] : czero<R>();
+#pragma unroll
+                for (int jj = 0; jj < SJ; ++jj) fma_c(acc[jj], c.a[t], v[jj]);
             }
+        if constexpr (KW > 0) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                if (u < c.nm) {
+                    const cplx<R>* W = c.W[u] + r;
+                    const int kw = c.kw[u];
+                    cplx<R> w[KW];
+#pragma unroll
+                    for (int l = 0; l < KW; ++l) w[l] = (l < kw) ? W[(int64_t)l * c.ldw[u]] : czero<R>();
+#pragma unroll
+                    for (int l = 0; l < KW; ++l)
+#pragma unroll
+                        for (int jj = 0; jj < SJ; ++jj) fma_c(acc[jj], w[l], Ms[u][l][jj]);
+                }
         }
 #pragma unroll
         for (int jj = 0; jj < SJ; ++jj)
@@ -204,32 +266,28 @@ __global__ __launch_bounds__(256) void k_combo(ComboDev<R> c, cplx<R>* Z, int64_
 
 template <typename R>
 __global__ void k_scale_rows(const cplx<R>* __restrict__ d, const cplx<R>* A, int64_t lda, cplx<R>* Z, int64_t ldz,
-                             int64_t rows, int s)
+                             int64_t rows)
 {
-    const int64_t total = rows * s;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = i % rows, j = i / rows;
+    const int64_t j = blockIdx.y;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x)
         Z[j * ldz + r] = A[j * lda + r] * d[r];
-    }
 }
 
 __global__ void k_cvt_64_32(const c64* __restrict__ A, int64_t lda, c32* __restrict__ Z, int64_t ldz, int64_t rows,
-                            int s, double scale)
+                            double scale)
 {
-    const int64_t total = rows * s;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = i % rows, j = i / rows;
+    const int64_t j = blockIdx.y;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
         const c64 v = A[j * lda + r];
         Z[j * ldz + r] = mk((float)(v.x * scale), (float)(v.y * scale));
     }
 }
 
 __global__ void k_acc_32_64(const c32* __restrict__ A, int64_t lda, c64* __restrict__ Z, int64_t ldz, int64_t rows,
-                            int s, double scale)
+                            double scale)
 {
-    const int64_t total = rows * s;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = i % rows, j = i / rows;
+    const int64_t j = blockIdx.y;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
         const c32 v = A[j * lda + r];
         c64 z = Z[j * ldz + r];
         z.x += scale * (double)v.x;
@@ -239,19 +297,20 @@ __global__ void k_acc_32_64(const c32* __restrict__ A, int64_t lda, c64* __restr
 }
 
 template <typename R>
-__global__ void k_zero(cplx<R>* Z, int64_t ldz, int64_t rows, int s)
+__global__ void k_zero(cplx<R>* Z, int64_t ldz, int64_t rows)
 {
-    const int64_t total = rows * s;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = i % rows, j = i / rows;
+    const int64_t j = blockIdx.y;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x)
         Z[j * ldz + r] = czero<R>();
-    }
 }
 
 // ------------------------------------------------------------- host
-static int grid_elem(int64_t n)
+// rows x s element kernels: blockIdx.y is the column
+static dim3 grid_cols(int64_t rows, int s)
 {
-    return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 8192);
+    s = std::max(s, 1);
+    const int64_t gx = std::min<int64_t>(std::max<int64_t>(ceil_div(rows, 256), 1), std::max(1, 16384 / s));
+    return dim3((unsigned)gx, (unsigned)s);
 }
 
 template <typename R>
@@ -259,15 +318,17 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
              int64_t rows, int conjA)
 {
     const int np = a * b;
-    if (a < 1 || b < 1 || np > 4096) throw HipError{hipErrorInvalidValue, "gram: block too wide"};
+    const int ntile = ((a + 3) / 4) * ((b + 3) / 4);
+    if (a < 1 || b < 1 || np > 4096 || ntile > NT_RED) throw HipError{hipErrorInvalidValue, "gram: block too wide"};
     const size_t c = sizeof(cplx<R>);
-    int TR = (int)std::min<int64_t>(256, 32768 / ((a + b) * c));
+    const int pitch = 4 * ((a + 3) / 4) + 4 * ((b + 3) / 4);
+    int TR = (int)std::min<int64_t>(128, 32768 / (pitch * c));
     TR = std::max(TR, 1);
     int64_t rpb = std::max<int64_t>(TR, ceil_div(std::max<int64_t>(rows, 1), MAX_PARTIAL_BLOCKS));
     rpb = ceil_div(rpb, TR) * TR;
     const int nblk = (int)std::max<int64_t>(1, ceil_div(rows, rpb));
     w.reserve((size_t)nblk * np, np);
-    const size_t shm = c * (size_t)TR * (a + b);
+    const size_t shm = std::max<size_t>(c * (size_t)TR * pitch, sizeof(double) * 2 * NT_RED);
     if (conjA)
         hipLaunchKernelGGL((k_gram_partial<R, true>), dim3(nblk), dim3(NT_RED), shm, w.stream, A, lda, a, B, ldb, b,
                            rows, TR, rpb, w.partial);
@@ -289,7 +350,7 @@ template <typename R>
 c64 dot_all(BlockWork& w, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb, int64_t rows, int s,
             int conjA)
 {
-    const int nblk = (int)std::min<int64_t>(MAX_PARTIAL_BLOCKS, std::max<int64_t>(1, ceil_div(rows * s, NT_RED)));
+    const int nblk = (int)std::min<int64_t>(MAX_PARTIAL_BLOCKS, std::max<int64_t>(1, ceil_div(rows, NT_RED)));
     w.reserve(nblk, 1);
     if (conjA)
         hipLaunchKernelGGL((k_dot_partial<R, true>), dim3(nblk), dim3(NT_RED), 0, w.stream, A, lda, B, ldb, rows, s,
@@ -312,6 +373,34 @@ double norm_fro(BlockWork& w, const cplx<R>* A, int64_t lda, int64_t rows, int s
     return std::sqrt(std::max(v.x, 0.0));
 }
 
+
+// column chunk width: wide coefficient blocks get narrower chunks so that
+// a row's KW inputs and SJ accumulators stay in registers
+template <typename R, int KW>
+static void launch_combo(hipStream_t st, const ComboDev<R>& cd, cplx<R>* Z, int64_t ldz, int64_t rows, int s)
+{
+    constexpr int SJMAX = KW <= 16 ? 16 : (KW == 32 ? 8 : 4);
+    int sj = SJMAX;
+    while (sj > 1 && sj / 2 >= s) sj /= 2;
+    const dim3 grid((unsigned)std::max<int64_t>(1, ceil_div(rows, 256)));
+    for (int j0 = 0; j0 < s; j0 += sj) {
+        const int nj = std::min(sj, s - j0);
+        switch (sj) {
+        case 1: hipLaunchKernelGGL((k_combo<R, KW, 1>), grid, dim3(256), 0, st, cd, Z, ldz, rows, j0, nj); break;
+        case 2: hipLaunchKernelGGL((k_combo<R, KW, 2>), grid, dim3(256), 0, st, cd, Z, ldz, rows, j0, nj); break;
+        case 4: hipLaunchKernelGGL((k_combo<R, KW, 4>), grid, dim3(256), 0, st, cd, Z, ldz, rows, j0, nj); break;
+        case 8:
+            if constexpr (SJMAX >= 8)
+                hipLaunchKernelGGL((k_combo<R, KW, 8>), grid, dim3(256), 0, st, cd, Z, ldz, rows, j0, nj);
+            break;
+        default:
+            if constexpr (SJMAX >= 16)
+                hipLaunchKernelGGL((k_combo<R, KW, 16>), grid, dim3(256), 0, st, cd, Z, ldz, rows, j0, nj);
+            break;
+        }
+        BDDA_HIP(hipGetLastError());
+    }
+}
 
 template <typename R>
 void combo(BlockWork& w, const Combo<R>& c, cplx<R>* Z, int64_t ldz, int64_t rows, int s)
@@ -346,45 +435,46 @@ void combo(BlockWork& w, const Combo<R>& c, cplx<R>* Z, int64_t ldz, int64_t row
         }
         BDDA_HIP(hipMemcpyAsync(dslot, hslot, sizeof(cplx<R>) * off, hipMemcpyHostToDevice, w.stream));
     }
-    constexpr int SJ = 16;
-    for (int j0 = 0; j0 < s; j0 += SJ) {
-        const int nj = std::min(SJ, s - j0);
-        if (nj <= 8 && j0 == 0 && s <= 8)
-            hipLaunchKernelGGL((k_combo<R, 8>), dim3(grid_elem(rows)), dim3(256), 0, w.stream, cd, Z, ldz, rows, j0,
-                               nj);
-        else
-            hipLaunchKernelGGL((k_combo<R, SJ>), dim3(grid_elem(rows)), dim3(256), 0, w.stream, cd, Z, ldz, rows, j0,
-                               nj);
-        BDDA_HIP(hipGetLastError());
-    }
+    int kwmax = 0;
+    for (int u = 0; u < c.nm; ++u) kwmax = std::max(kwmax, c.kw[u]);
+    if (kwmax == 0) launch_combo<R, 0>(w.stream, cd, Z, ldz, rows, s);
+    else if (kwmax <= 4) launch_combo<R, 4>(w.stream, cd, Z, ldz, rows, s);
+    else if (kwmax <= 8) launch_combo<R, 8>(w.stream, cd, Z, ldz, rows, s);
+    else if (kwmax <= 16) launch_combo<R, 16>(w.stream, cd, Z, ldz, rows, s);
+    else if (kwmax <= 32) launch_combo<R, 32>(w.stream, cd, Z, ldz, rows, s);
+    else launch_combo<R, 64>(w.stream, cd, Z, ldz, rows, s);
 }
 
 template <typename R>
 void scale_rows(hipStream_t st, const cplx<R>* d, const cplx<R>* A, int64_t lda, cplx<R>* Z, int64_t ldz,
                 int64_t rows, int s)
 {
-    hipLaunchKernelGGL(k_scale_rows<R>, dim3(grid_elem(rows * s)), dim3(256), 0, st, d, A, lda, Z, ldz, rows, s);
+    if (rows <= 0 || s <= 0) return;
+    hipLaunchKernelGGL(k_scale_rows<R>, grid_cols(rows, s), dim3(256), 0, st, d, A, lda, Z, ldz, rows);
     BDDA_HIP(hipGetLastError());
 }
 
 void convert_64_to_32(hipStream_t st, const c64* A, int64_t lda, c32* Z, int64_t ldz, int64_t rows, int s,
                       double scale)
 {
-    hipLaunchKernelGGL(k_cvt_64_32, dim3(grid_elem(rows * s)), dim3(256), 0, st, A, lda, Z, ldz, rows, s, scale);
+    if (rows <= 0 || s <= 0) return;
+    hipLaunchKernelGGL(k_cvt_64_32, grid_cols(rows, s), dim3(256), 0, st, A, lda, Z, ldz, rows, scale);
     BDDA_HIP(hipGetLastError());
 }
 
 void accumulate_32_to_64(hipStream_t st, const c32* A, int64_t lda, c64* Z, int64_t ldz, int64_t rows, int s,
                          double scale)
 {
-    hipLaunchKernelGGL(k_acc_32_64, dim3(grid_elem(rows * s)), dim3(256), 0, st, A, lda, Z, ldz, rows, s, scale);
+    if (rows <= 0 || s <= 0) return;
+    hipLaunchKernelGGL(k_acc_32_64, grid_cols(rows, s), dim3(256), 0, st, A, lda, Z, ldz, rows, scale);
     BDDA_HIP(hipGetLastError());
 }
 
 template <typename R>
 void fill_zero(hipStream_t st, cplx<R>* Z, int64_t ldz, int64_t rows, int s)
 {
-    hipLaunchKernelGGL(k_zero<R>, dim3(grid_elem(rows * s)), dim3(256), 0, st, Z, ldz, rows, s);
+    if (rows <= 0 || s <= 0) return;
+    hipLaunchKernelGGL(k_zero<R>, grid_cols(rows, s), dim3(256), 0, st, Z, ldz, rows);
     BDDA_HIP(hipGetLastError());
 }
 

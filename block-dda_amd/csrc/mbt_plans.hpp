@@ -47,11 +47,44 @@ inline bool has_plan(int P)
     return with_plan(P, [](auto) {});
 }
 
+// Threads per group for NQ lines of a plan.  A stage of radix r has
+// P / r * NQ butterflies, spread over the group in rounds; the choice is the
+// multiple of 64 that leaves the fewest lanes idle over all stages, with at
+// most MB rounds per stage (each round's values live in registers), ties
+// going to the size nearest TARGET.  343 = 7^3 with 16 lines (784
+// butterflies per stage) gets 832 threads, one butterfly each, instead of
+// four rounds of 256 with the last one a quarter full.
+template <class PL, int NQ, int MB, int TARGET>
+constexpr int plan_threads()
+{
+    int best = 0;
+    long best_used = 0, best_slots = 1;
+    for (int nt = 64; nt <= 1024; nt += 64) {
+        long used = 0, slots = 0;
+        bool fits = true;
+        for (int s = 0; s < PL::N; ++s) {
+            const long b = (long)(PL::P / PL::r[s]) * NQ;
+            const long rounds = (b + nt - 1) / nt;
+            fits = fits && rounds <= MB;
+            used += b;
+            slots += rounds * nt;
+        }
+        if (!fits) continue;
+        const long lhs = used * best_slots, rhs = best_used * slots;
+        const int dn = nt > TARGET ? nt - TARGET : TARGET - nt;
+        const int db = best > TARGET ? best - TARGET : TARGET - best;
+        if (best == 0 || lhs > rhs || (lhs == rhs && dn < db)) {
+            best = nt;
+            best_used = used;
+            best_slots = slots;
+        }
+    }
+    return best == 0 ? 1024 : best;
+}
+
 // tile widths of the planned passes: 128-byte row segments in every pass
 template <typename R>
 struct PlanTiles {
-    static constexpr int NT = 256;
-    static constexpr int NT_X = 512;
     static constexpr int NQ_Y = sizeof(R) == 4 ? 16 : 8;  // y pass: columns per group
     static constexpr int NQ_X = sizeof(R) == 4 ? 16 : 8;  // x pass: kz per group (all F planes)
     static constexpr int TL_Z = sizeof(R) == 4 ? 8 : 4;   // z pass: lines per group
@@ -264,7 +297,8 @@ inline bool launch_lines_plan(const cplx<R>* in, cplx<R>* out, int nlin, int nlo
         using PL = decltype(pl);
         const int64_t ntiles = ceil_div(Zdim, T::NQ_Y);
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_Y);
-        hipLaunchKernelGGL((k_lines_plan<R, PL, INV, T::NT, T::NQ_Y>), dim3((unsigned)(ntiles * outer)), dim3(T::NT),
+        constexpr int NT = plan_threads<PL, T::NQ_Y, 4, 256>();
+        hipLaunchKernelGGL((k_lines_plan<R, PL, INV, NT, T::NQ_Y>), dim3((unsigned)(ntiles * outer)), dim3(NT),
                            shm, st, in, out, nlin, nlout, Zdim, pitch, ntiles, tw);
         BDDA_HIP(hipGetLastError());
     });
@@ -279,8 +313,9 @@ inline bool launch_xfused_plan(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, 
         using PL = decltype(pl);
         const int ntz = (int)ceil_div(d.Pz, T::NQ_X);
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_X);
-        hipLaunchKernelGGL((k_xfused_plan<R, PL, F, T::NT_X, T::NQ_X>), dim3((unsigned)((int64_t)s * d.Py * ntz)),
-                           dim3(T::NT_X), shm, st, V, gh, d, Pzs, tw, ntz);
+        constexpr int NT = plan_threads<PL, T::NQ_X, 2, 512>();
+        hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, T::NQ_X>), dim3((unsigned)((int64_t)s * d.Py * ntz)),
+                           dim3(NT), shm, st, V, gh, d, Pzs, tw, ntz);
         BDDA_HIP(hipGetLastError());
     });
 }
@@ -295,7 +330,8 @@ inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx
         const int64_t nxny = (int64_t)d.nx * d.ny;
         const int64_t ntiles = ceil_div(nxny, T::TL_Z);
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::TL_Z * F);
-        hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, T::NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(T::NT), shm, st,
+        constexpr int NT = plan_threads<PL, T::TL_Z, 2, 256>();
+        hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
                            X, ldx, c2o, U, d, Pzs, tw, ntiles);
         BDDA_HIP(hipGetLastError());
     });
@@ -312,7 +348,8 @@ inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X,
         const int64_t nxny = (int64_t)d.nx * d.ny;
         const int64_t ntiles = ceil_div(nxny, T::TL_Z);
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::TL_Z * F);
-        hipLaunchKernelGGL((k_zinv_plan<R, PL, F, T::NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(T::NT), shm, st,
+        constexpr int NT = plan_threads<PL, T::TL_Z, 2, 256>();
+        hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
                            U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles);
         BDDA_HIP(hipGetLastError());
     });
