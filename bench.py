@@ -161,6 +161,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--matvec-reps", type=int, default=5)
+    ap.add_argument("--matvec-only", action="store_true",
+                    help="time the block matvecs and exit (for counter runs)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also run one device-to-device copy of this many bytes: the known-size "
+                         "reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solve path is measured on a single MI355X "
@@ -199,6 +204,20 @@ def main():
         log("matvec %s: %s  total %.2f ms (%.0f GB/s algorithmic)" % (
             name, {kk: round(v, 3) for kk, v in ms.items()}, sum(ms.values()), mv[name]["matvec_GBs"]))
 
+    if args.copy_bytes > 0:
+        cA, cB = ctypes.c_void_p(), ctypes.c_void_p()
+        pkg._check(L.bdda_dev_alloc(args.copy_bytes, ctypes.byref(cA)))
+        pkg._check(L.bdda_dev_alloc(args.copy_bytes, ctypes.byref(cB)))
+        pkg._check(L.bdda_memcpy_d2d(cB, cA, args.copy_bytes))
+        pkg._check(L.bdda_device_sync())
+        L.bdda_dev_free(cA)
+        L.bdda_dev_free(cB)
+        log("copy: %d bytes device to device" % args.copy_bytes)
+    if args.matvec_only:
+        print(json.dumps({"matvec": {kk: {"pass_ms": v["pass_ms"], "matvec_GBs": v["matvec_GBs"],
+                                          "bytes": v["bytes"]} for kk, v in mv.items()},
+                          "copy_bytes": args.copy_bytes}), flush=True)
+        return
     # FP64 reference-semantics solve (recurrence residual < tol)
     fp64 = None
     if not args.no_fp64:

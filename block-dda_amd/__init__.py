@@ -85,6 +85,7 @@ SIGNATURES = {
     "bdda_dev_free": (ctypes.c_int, [_vp]),
     "bdda_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "bdda_memcpy_d2h": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "bdda_memcpy_d2d": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "bdda_device_sync": (ctypes.c_int, []),
     "bdda_solve_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_int, _vp, ctypes.POINTER(SolveInfo), _dp,

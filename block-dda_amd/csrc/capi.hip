@@ -416,6 +416,12 @@ int bdda_memcpy_d2h(void* dst, const void* src, size_t bytes)
     return e == hipSuccess ? BDDA_OK : fail(BDDA_EHIP, hipGetErrorString(e));
 }
 
+int bdda_memcpy_d2d(void* dst, const void* src, size_t bytes)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
+    return e == hipSuccess ? BDDA_OK : fail(BDDA_EHIP, hipGetErrorString(e));
+}
+
 int bdda_device_sync(void)
 {
     hipError_t e = hipDeviceSynchronize();

@@ -82,6 +82,27 @@ constexpr int plan_threads()
     return best == 0 ? 1024 : best;
 }
 
+// Work-group -> (tile, right-hand side).  Work-groups are dispatched round-
+// robin over the 8 XCDs (group b runs on XCD b mod 8), each with its own L2.
+// Tiles are dealt to XCDs in the same round-robin, and on each XCD the s
+// right-hand sides of a tile run back to back, so the per-tile data they
+// share (G^, the occupancy map, DIAG) comes from HBM once per tile and from
+// that XCD's L2 for the other s - 1.  A bijection on [0, ntiles * s).
+__device__ __forceinline__ void xcd_tile_rhs(int64_t b, int64_t ntiles, int s, int64_t& tile, int& rhs)
+{
+    constexpr int NXCD = 8;
+    const int64_t full = (ntiles / NXCD) * NXCD * s;
+    if (b < full) {
+        const int64_t local = b / NXCD;
+        tile = (local / s) * NXCD + b % NXCD;
+        rhs = (int)(local % s);
+    } else {
+        const int64_t r = b - full;
+        tile = (ntiles / NXCD) * NXCD + r / s;
+        rhs = (int)(r % s);
+    }
+}
+
 // tile widths of the planned passes: 128-byte row segments in every pass
 template <typename R>
 struct PlanTiles {
@@ -123,7 +144,8 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
 // A group owns (rhs, ky, NQ consecutive kz) across all F component planes.
 template <typename R, class PL, int F, int NT, int NQ>
 __global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, const cplx<R>* __restrict__ Gh,
-                                                    MbtDims d, int Pzs, const cplx<R>* __restrict__ twg, int ntz)
+                                                    MbtDims d, int Pzs, const cplx<R>* __restrict__ twg, int ntz,
+                                                    int nrhs)
 {
     constexpr int P = PL::P;
     constexpr int NC = (F == 3) ? 6 : 9;
@@ -132,11 +154,11 @@ __global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, con
     cplx<R>* lds = tw + P;
     const int64_t plane = (int64_t)d.Py * Pzs;
     const int64_t cstride = (int64_t)d.nx * plane;
-    const int64_t per_rhs = (int64_t)d.Py * ntz;
-    const int rhs = (int)(blockIdx.x / per_rhs);
-    const int rem = (int)(blockIdx.x % per_rhs);
-    const int ky = rem / ntz;
-    const int kz0 = (rem % ntz) * NQ;
+    int64_t tile;
+    int rhs;
+    xcd_tile_rhs(blockIdx.x, (int64_t)d.Py * ntz, nrhs, tile, rhs);
+    const int ky = (int)(tile / ntz);
+    const int kz0 = (int)(tile % ntz) * NQ;
     const int tqn = min(NQ, d.Pz - kz0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     cplx<R>* base = V + (int64_t)rhs * F * cstride + (int64_t)ky * Pzs + kz0;
@@ -196,15 +218,17 @@ __global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, con
 template <typename R, class PL, int F, int NT, int TL>
 __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X, int64_t ldx,
                                                   const int* __restrict__ c2o, cplx<R>* __restrict__ U, MbtDims d,
-                                                  int Pzs, const cplx<R>* __restrict__ twg, int64_t ntiles)
+                                                  int Pzs, const cplx<R>* __restrict__ twg, int64_t ntiles, int nrhs)
 {
     constexpr int P = PL::P;
     extern __shared__ unsigned char smem_raw[];
     cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* lds = tw + P;
     const int64_t nxny = (int64_t)d.nx * d.ny;
-    const int rhs = (int)(blockIdx.x / ntiles);
-    const int64_t l0 = (blockIdx.x % ntiles) * (int64_t)TL;
+    int64_t tile;
+    int rhs;
+    xcd_tile_rhs(blockIdx.x, ntiles, nrhs, tile, rhs);
+    const int64_t l0 = tile * TL;
     const int nl = (int)min((int64_t)TL, nxny - l0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     const cplx<R>* Xr = X + rhs * ldx;
@@ -241,15 +265,17 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
                                                   const cplx<R>* __restrict__ X, int64_t ldx,
                                                   const cplx<R>* __restrict__ diag, const cplx<R>* __restrict__ jpre,
                                                   cplx<R>* __restrict__ Y, int64_t ldy, MbtDims d, int Pzs,
-                                                  const cplx<R>* __restrict__ twg, int64_t ntiles)
+                                                  const cplx<R>* __restrict__ twg, int64_t ntiles, int nrhs)
 {
     constexpr int P = PL::P;
     extern __shared__ unsigned char smem_raw[];
     cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* lds = tw + P;
     const int64_t nxny = (int64_t)d.nx * d.ny;
-    const int rhs = (int)(blockIdx.x / ntiles);
-    const int64_t l0 = (blockIdx.x % ntiles) * (int64_t)TL;
+    int64_t tile;
+    int rhs;
+    xcd_tile_rhs(blockIdx.x, ntiles, nrhs, tile, rhs);
+    const int64_t l0 = tile * TL;
     const int nl = (int)min((int64_t)TL, nxny - l0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     for (int e = threadIdx.x; e < F * TL * P; e += NT) {
@@ -315,7 +341,7 @@ inline bool launch_xfused_plan(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, 
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_X);
         constexpr int NT = plan_threads<PL, T::NQ_X, 2, 512>();
         hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, T::NQ_X>), dim3((unsigned)((int64_t)s * d.Py * ntz)),
-                           dim3(NT), shm, st, V, gh, d, Pzs, tw, ntz);
+                           dim3(NT), shm, st, V, gh, d, Pzs, tw, ntz, s);
         BDDA_HIP(hipGetLastError());
     });
 }
@@ -332,7 +358,7 @@ inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::TL_Z * F);
         constexpr int NT = plan_threads<PL, T::TL_Z, 2, 256>();
         hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
-                           X, ldx, c2o, U, d, Pzs, tw, ntiles);
+                           X, ldx, c2o, U, d, Pzs, tw, ntiles, s);
         BDDA_HIP(hipGetLastError());
     });
 }
@@ -350,7 +376,7 @@ inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X,
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::TL_Z * F);
         constexpr int NT = plan_threads<PL, T::TL_Z, 2, 256>();
         hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
-                           U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles);
+                           U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles, s);
         BDDA_HIP(hipGetLastError());
     });
 }

@@ -126,6 +126,8 @@ int bdda_dev_alloc(size_t bytes, void** p);
 int bdda_dev_free(void* p);
 int bdda_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int bdda_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* device-to-device copy (also the known-size reference for HBM counters) */
+int bdda_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int bdda_device_sync(void);
 int bdda_solve_dev(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64* dB, int L, double tol,
                    int itermax, bdda_c64* dX, bdda_solve_info* info, double* hist, int hist_len);
