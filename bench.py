@@ -150,6 +150,38 @@ def cpu_baseline(tgt, k, diag, s, n_matvecs, threads):
     }
 
 
+def observables_dev(pkg, op, dB, dX, waves, s, nang=181):
+    """Cross sections (all columns) and S11/S12 (one plane per incident
+    direction) of the device block dX, FP64 on the GPU."""
+    L = pkg.lib()
+    out = [np.zeros(s) for _ in range(3)]
+    pkg._check(L.bdda_cross_sections_dev(op.h, dB, dX, s, 0, *[o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                                                               for o in out]))
+    u = np.ascontiguousarray(waves.dirs[0::2])
+    e1 = np.ascontiguousarray(waves.pols[0::2].real)
+    npairs = u.shape[0]
+    S11 = np.zeros((npairs, nang))
+    S12 = np.zeros((npairs, nang))
+    dp = ctypes.POINTER(ctypes.c_double)
+    pkg._check(L.bdda_scattering_matrix_dev(op.h, dX, npairs, u.ctypes.data_as(dp), e1.ctypes.data_as(dp), nang,
+                                            S11.ctypes.data_as(dp), S12.ctypes.data_as(dp), None))
+    return {"Cext": out[0], "Cabs": out[1], "Csca": out[2], "S11": S11, "S12": S12}
+
+
+def compare_observables(mx, ref, geom_cs):
+    """Errors of the mixed-precision observables against the FP64 solve's."""
+    res = {}
+    for q in ("Cext", "Cabs", "Csca"):
+        res["Q" + q[1:] + "_fp64_mean"] = float(np.mean(ref[q]) / geom_cs)
+        res["Q" + q[1:] + "_mixed_mean"] = float(np.mean(mx[q]) / geom_cs)
+        res[q + "_max_rel_err"] = float(np.max(np.abs(mx[q] - ref[q]) / np.abs(ref[q])))
+    scale = np.max(ref["S11"])
+    res["S11_max_err_rel_to_max"] = float(np.max(np.abs(mx["S11"] - ref["S11"])) / scale)
+    res["S12_max_err_rel_to_S11max"] = float(np.max(np.abs(mx["S12"] - ref["S12"])) / scale)
+    res["angles"] = int(ref["S11"].shape[1])
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,9 +252,11 @@ def main():
         return
     # FP64 reference-semantics solve (recurrence residual < tol)
     fp64 = None
+    obs64 = None
     if not args.no_fp64:
         fp64 = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_FP64)
         log("fp64 solve: %s" % fp64)
+        obs64 = observables_dev(pkg, op, dB, dX, W.waves, s)
     # mixed precision (true FP64 residual <= tol): warmup + timed steps
     for _ in range(args.warmup):
         r = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_MIXED)
@@ -235,6 +269,12 @@ def main():
         log("mixed step: %s" % runs[-1])
     pkg._check(L.bdda_device_sync())
     total = time.perf_counter() - t_all
+    obs = None
+    if obs64 is not None:
+        aeff = (3.0 * tgt.nocc * tgt.lf ** 3 / (4.0 * math.pi)) ** (1.0 / 3.0)
+        obs = compare_observables(observables_dev(pkg, op, dB, dX, W.waves, s), obs64, math.pi * aeff ** 2)
+        obs["a_eff"] = aeff
+        log("observables (mixed vs fp64): %s" % obs)
     ms_per_step = total / max(args.steps, 1) * 1e3
     tts = ms_per_step / 1e3
 
@@ -265,6 +305,7 @@ def main():
         "fp64": {k2: fp64[k2] for k2 in ("iters", "matvecs", "true_res", "recurrence_res", "converged")}
         if fp64 else None,
         "speedup_mixed_vs_fp64": (fp64["wall_s"] / tts) if fp64 else None,
+        "observables": obs,
         "init_s": init_s,
         "matvec": {kk: {"pass_ms": v["pass_ms"], "matvec_GBs": v["matvec_GBs"],
                         "pass_GBs": v["pass_GBs"]} for kk, v in mv.items()},

@@ -94,7 +94,14 @@ SIGNATURES = {
     "bdda_mbt_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "bdda_mbt_pass_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "bdda_mbt_algorithmic_bytes": (ctypes.c_double, [_vp, ctypes.c_int, ctypes.c_int]),
+    "bdda_cross_sections": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
+    "bdda_cross_sections_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
+    "bdda_scattering_matrix": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp,
+                                              _vp]),
+    "bdda_scattering_matrix_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _vp]),
 }
+
+ALPHA_FIRST, ALPHA_EACH = 0, 1
 
 
 def lib():
@@ -244,6 +251,34 @@ class MbtOperator:
         if rc not in (0, 5):
             _check(rc)
         return X, info.as_dict(), hist[:hist_len]
+
+
+    def cross_sections(self, B, X, alpha_mode=ALPHA_FIRST):
+        """Cext, Cabs, Csca per column (Result_manager.cpp:65-122), FP64 on the GPU."""
+        B = _cz(B).reshape(B.shape[0], -1, order="F")
+        X = _cz(X).reshape(X.shape[0], -1, order="F")
+        L = B.shape[1]
+        out = [np.zeros(L) for _ in range(3)]
+        _check(lib().bdda_cross_sections(self.h, _ptr(B), B.shape[0], _ptr(X), X.shape[0], L, int(alpha_mode),
+                                         *[o.ctypes.data_as(_dp) for o in out]))
+        return {"Cext": out[0], "Cabs": out[1], "Csca": out[2]}
+
+    def scattering_matrix(self, X, u, e1, nang=181):
+        """S11, S12 (npairs x nang) and S1..S4 (npairs x nang x 4) of column pairs
+        (2p: polarisation e1_p, 2p+1: u_p x e1_p) in the plane of u_p and e1_p."""
+        X = _cz(X).reshape(X.shape[0], -1, order="F")
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 3)
+        e1 = np.ascontiguousarray(e1, dtype=np.float64).reshape(-1, 3)
+        npairs = u.shape[0]
+        if X.shape[1] < 2 * npairs or e1.shape[0] != npairs:
+            raise BddaError(1, "need two columns of X and one e1 per direction")
+        S11 = np.zeros((npairs, nang))
+        S12 = np.zeros((npairs, nang))
+        S = np.zeros((npairs, nang, 4), dtype=np.complex128)
+        _check(lib().bdda_scattering_matrix(self.h, _ptr(X), X.shape[0], npairs, u.ctypes.data_as(_dp),
+                                            e1.ctypes.data_as(_dp), int(nang), S11.ctypes.data_as(_dp),
+                                            S12.ctypes.data_as(_dp), _ptr(S)))
+        return S11, S12, S
 
 
 def MBT_fft_init(n, f, lf, k, plan_fwd=None):

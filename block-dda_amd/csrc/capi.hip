@@ -1,5 +1,6 @@
 // capi.hip -- extern "C" entry points declared in include/bdda.h.
 #include "../../include/bdda.h"
+#include "observables.hpp"
 #include "solvers.hpp"
 #include <cmath>
 #include <complex>
@@ -171,6 +172,70 @@ static int solve_common(bdda_mbt* h, int method, int jacobi, int mode, const c64
     else if (rc == BDDA_ENOCONV) set_error("itermax reached before tol");
     return rc;
 }
+
+// --------------------------------------------------------------- observables
+static int obs_check(const bdda_mbt* h, int L, double* kout)
+{
+    if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
+    if (!h->op->have64) return fail(BDDA_EINVAL, "observables need the FP64 operator");
+    if (h->op->kim != 0.0) return fail(BDDA_EINVAL, "non zero imag(k) is not supported");
+    if (L < 1) return fail(BDDA_EINVAL, "need at least one column");
+    *kout = h->op->kre;
+    return BDDA_OK;
+}
+
+static int xs_dev(bdda_mbt* h, const c64* dB, const c64* dX, int L, int alpha_mode, double* Cext, double* Cabs,
+                  double* Csca)
+{
+    double k = 0;
+    int rc = obs_check(h, L, &k);
+    if (rc) return rc;
+    if (alpha_mode != BDDA_ALPHA_FIRST && alpha_mode != BDDA_ALPHA_EACH) return fail(BDDA_EINVAL, "bad alpha_mode");
+    if (!Cext || !Cabs || !Csca) return fail(BDDA_EINVAL, "null pointer");
+    cross_sections(h->prob, dB, dX, L, k, alpha_mode == BDDA_ALPHA_EACH, Cext, Cabs);
+    for (int l = 0; l < L; ++l) Csca[l] = Cext[l] - Cabs[l];
+    return BDDA_OK;
+}
+
+static int sm_dev(bdda_mbt* h, const c64* dX, int npairs, const double* u, const double* e1, int nang, double* S11,
+                  double* S12, bdda_c64* S)
+{
+    double k = 0;
+    int rc = obs_check(h, 2 * npairs, &k);
+    if (rc) return rc;
+    if (!u || !e1 || !S11 || !S12) return fail(BDDA_EINVAL, "null pointer");
+    if (npairs < 1 || nang < 2) return fail(BDDA_EINVAL, "need npairs >= 1 and nang >= 2");
+    for (int p = 0; p < npairs; ++p) {
+        const double* a = u + 3 * p;
+        const double* b = e1 + 3 * p;
+        const double na = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+        const double nb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+        const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+        if (std::fabs(na - 1) > 1e-9 || std::fabs(nb - 1) > 1e-9 || std::fabs(ab) > 1e-9)
+            return fail(BDDA_EINVAL, "u and e1 must be orthonormal");
+    }
+    std::vector<c64> amp((size_t)npairs * nang * 4);
+    scattering_amplitudes(h->prob, dX, npairs, k, u, e1, nang, amp.data());
+    for (int64_t e = 0; e < (int64_t)npairs * nang; ++e) {
+        double m2[4];
+        for (int q = 0; q < 4; ++q) m2[q] = amp[e * 4 + q].x * amp[e * 4 + q].x + amp[e * 4 + q].y * amp[e * 4 + q].y;
+        S11[e] = 0.5 * (m2[0] + m2[1] + m2[2] + m2[3]);
+        S12[e] = 0.5 * (m2[1] - m2[0] + m2[3] - m2[2]);
+        if (S)
+            for (int q = 0; q < 4; ++q) S[e * 4 + q] = {amp[e * 4 + q].x, amp[e * 4 + q].y};
+    }
+    return BDDA_OK;
+}
+
+template <typename T>
+struct DevMem {
+    T* p = nullptr;
+    explicit DevMem(size_t n) { BDDA_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1))); }
+    ~DevMem()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
 
 extern "C" {
 
@@ -361,6 +426,53 @@ int bdda_solve_dev(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64
         return solve_common(h, method, jacobi, mode, reinterpret_cast<const c64*>(dB), L, tol, itermax,
                             reinterpret_cast<c64*>(dX), info, hist, hist_len);
     })
+}
+
+int bdda_cross_sections(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdda_c64* X, int64_t ldx, int L,
+                        int alpha_mode, double* Cext, double* Cabs, double* Csca)
+{
+    if (!h || !B || !X) return fail(BDDA_EINVAL, "null pointer");
+    BDDA_TRY({
+        if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
+        const int64_t rows = h->prob.rows();
+        if (L < 1 || ldb < rows || ldx < rows) return fail(BDDA_EINVAL, "bad block shape");
+        DevMem<c64> dB((size_t)rows * L);
+        DevMem<c64> dX((size_t)rows * L);
+        to_dev<double>(B, ldb, rows, L, dB.p, h->stream);
+        to_dev<double>(X, ldx, rows, L, dX.p, h->stream);
+        return xs_dev(h, dB.p, dX.p, L, alpha_mode, Cext, Cabs, Csca);
+    })
+}
+
+int bdda_cross_sections_dev(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX, int L, int alpha_mode, double* Cext,
+                            double* Cabs, double* Csca)
+{
+    if (!h || !dB || !dX) return fail(BDDA_EINVAL, "null pointer");
+    BDDA_TRY({
+        return xs_dev(h, reinterpret_cast<const c64*>(dB), reinterpret_cast<const c64*>(dX), L, alpha_mode, Cext,
+                      Cabs, Csca);
+    })
+}
+
+int bdda_scattering_matrix(bdda_mbt* h, const bdda_c64* X, int64_t ldx, int npairs, const double* u,
+                           const double* e1, int nang, double* S11, double* S12, bdda_c64* S)
+{
+    if (!h || !X) return fail(BDDA_EINVAL, "null pointer");
+    BDDA_TRY({
+        if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
+        const int64_t rows = h->prob.rows();
+        if (npairs < 1 || ldx < rows) return fail(BDDA_EINVAL, "bad block shape");
+        DevMem<c64> dX((size_t)rows * 2 * npairs);
+        to_dev<double>(X, ldx, rows, 2 * npairs, dX.p, h->stream);
+        return sm_dev(h, dX.p, npairs, u, e1, nang, S11, S12, S);
+    })
+}
+
+int bdda_scattering_matrix_dev(bdda_mbt* h, const bdda_c64* dX, int npairs, const double* u, const double* e1,
+                               int nang, double* S11, double* S12, bdda_c64* S)
+{
+    if (!h || !dX) return fail(BDDA_EINVAL, "null pointer");
+    BDDA_TRY({ return sm_dev(h, reinterpret_cast<const c64*>(dX), npairs, u, e1, nang, S11, S12, S); })
 }
 
 int bdda_qr_reduced(const bdda_c64* A, int64_t m, int n, bdda_c64* Q, bdda_c64* R)

@@ -250,6 +250,7 @@ void mbt_destroy(MbtOperator* op)
     if (op->gh32) (void)hipFree(op->gh32);
     if (op->ws) (void)hipFree(op->ws);
     if (op->c2o) (void)hipFree(op->c2o);
+    if (op->addr_dev) (void)hipFree(op->addr_dev);
     for (auto& e : op->ev)
         if (e) (void)hipEventDestroy(e);
     delete op;
@@ -262,15 +263,19 @@ void mbt_set_address(MbtOperator* op, const int* address_host, int64_t nocc)
         BDDA_HIP(hipMalloc(&op->c2o, sizeof(int) * C));
         op->c2o_cells = C;
     }
-    int* daddr = nullptr;
-    BDDA_HIP(hipMalloc(&daddr, sizeof(int) * std::max<int64_t>(nocc, 1)));
-    BDDA_HIP(hipMemcpyAsync(daddr, address_host, sizeof(int) * nocc, hipMemcpyHostToDevice, op->stream));
+    if (op->addr_n < nocc) {
+        if (op->addr_dev) BDDA_HIP(hipFree(op->addr_dev));
+        op->addr_dev = nullptr;
+        op->addr_n = 0;
+        BDDA_HIP(hipMalloc(&op->addr_dev, sizeof(int) * std::max<int64_t>(nocc, 1)));
+        op->addr_n = nocc;
+    }
+    BDDA_HIP(hipMemcpyAsync(op->addr_dev, address_host, sizeof(int) * nocc, hipMemcpyHostToDevice, op->stream));
     hipLaunchKernelGGL(k_fill_int, dim3(grid_for(C)), dim3(256), 0, op->stream, op->c2o, C, -1);
-    hipLaunchKernelGGL(k_build_c2o, dim3(grid_for(nocc)), dim3(256), 0, op->stream, (const int*)daddr, nocc,
+    hipLaunchKernelGGL(k_build_c2o, dim3(grid_for(nocc)), dim3(256), 0, op->stream, (const int*)op->addr_dev, nocc,
                        op->c2o);
     BDDA_HIP(hipGetLastError());
     BDDA_HIP(hipStreamSynchronize(op->stream));
-    BDDA_HIP(hipFree(daddr));
     op->c2o_host_addr.assign(address_host, address_host + nocc);
 }
 

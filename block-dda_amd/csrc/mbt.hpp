@@ -57,6 +57,8 @@ struct MbtOperator {
     // occupancy map for the occupied-vector path
     int* c2o = nullptr;         // cell -> occupied index or -1
     int64_t c2o_cells = 0;
+    int* addr_dev = nullptr;    // occupied index -> cell (the address array)
+    int64_t addr_n = 0;
     std::vector<int> c2o_host_addr;  // address array the map was built from
     // launch timing of the last apply (ms per pass), filled when timing on
     int timing = 0;

@@ -103,12 +103,17 @@ __device__ __forceinline__ void xcd_tile_rhs(int64_t b, int64_t ntiles, int s, i
     }
 }
 
-// tile widths of the planned passes: 128-byte row segments in every pass
+// tile widths of the planned passes (row segments of 32 to 128 bytes)
 template <typename R>
 struct PlanTiles {
     static constexpr int NQ_Y = sizeof(R) == 4 ? 16 : 8;  // y pass: columns per group
-    static constexpr int NQ_X = sizeof(R) == 4 ? 16 : 8;  // x pass: kz per group (all F planes)
-    static constexpr int TL_Z = sizeof(R) == 4 ? 8 : 4;   // z pass: lines per group
+    // x pass: kz per group (all F planes).  Eight FP32 / four FP64 lines
+    // (64-byte segments) keep three or more groups per CU; sixteen lines
+    // measured 11-15 % slower at 343^3 for lack of resident waves.
+    static constexpr int NQ_X = sizeof(R) == 4 ? 8 : 4;
+    // z pass: lines per group (the group's lines x F components x P sit in
+    // LDS for the gather / scatter transposition)
+    static constexpr int TL_Z = sizeof(R) == 4 ? 4 : 2;
 };
 
 // ---------------------------------------------------------------- y pass

@@ -119,6 +119,40 @@ int bdda_solve(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64* B,
  * coefficients, R(k,k) the norm after re-orthogonalisation). */
 int bdda_qr_reduced(const bdda_c64* A, int64_t m, int n, bdda_c64* Q, bdda_c64* R);
 
+/* ---------------------------------------------------------------- observables
+ * Cross sections of a solved block (Result_manager.cpp:65-122), on the GPU
+ * in FP64.  B is the incident-field block the solve used (EHinc) and X the
+ * dipole block (PM), both (f*nocc) x L; the wavenumber is the operator's and
+ * must be real (Result_manager.cpp:71-76).  Per column l:
+ *   Cext = 4 pi k Im(sum_i conj(B_il) X_il)
+ *   Cabs = 4 pi k sum_i (Im(conj(1/alpha_i)) - 2 k^3 / 3) |X_il|^2,
+ *          1/alpha_i = DIAG_A_i of the target
+ *   Csca = Cext - Cabs.
+ * BDDA_ALPHA_FIRST takes every dipole's alpha from dipole 0, as the
+ * reference does (Result_manager.cpp:73,97-98); BDDA_ALPHA_EACH uses each
+ * dipole's own DIAG_A (targets of several materials). */
+enum { BDDA_ALPHA_FIRST = 0, BDDA_ALPHA_EACH = 1 };
+int bdda_cross_sections(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdda_c64* X, int64_t ldx, int L,
+                        int alpha_mode, double* Cext, double* Cabs, double* Csca);
+
+/* Far-field scattering matrix.  No reference counterpart: the reference
+ * writes cross sections only.  Columns 2p and 2p+1 of X are the dipoles
+ * excited by plane waves along u_p polarised along e1_p and u_p x e1_p
+ * (u, e1: npairs x 3, orthonormal).  In the plane of u_p and e1_p, at
+ * theta_a = pi a / (nang - 1), the Bohren-Huffman amplitudes follow from
+ *   E_far = k^2 e^{ikr}/r sum_j [(I - n n) P_j - n x M_j] e^{-ik n.r_j},
+ *   n = cos(theta) u + sin(theta) e1, r_j = lf (ix, iy, iz):
+ *   S2 = -ik theta.E', S4 = -ik phi.E' (polarisation e1),
+ *   S3 = -ik theta.E', S1 = -ik phi.E' (polarisation u x e1),
+ * with E' = E_far r e^{-ikr}, phi = u x e1 (M_j for f = 6 only), and
+ *   S11 = (|S1|^2 + |S2|^2 + |S3|^2 + |S4|^2) / 2,
+ *   S12 = (|S2|^2 - |S1|^2 + |S4|^2 - |S3|^2) / 2.
+ * S11, S12: npairs x nang (pair-major); S (optional): npairs x nang x 4
+ * complex (S1, S2, S3, S4).  The forward amplitude obeys the optical theorem
+ * against bdda_cross_sections: Cext(pol e1) = 4 pi / k^2 Re S2(0). */
+int bdda_scattering_matrix(bdda_mbt* h, const bdda_c64* X, int64_t ldx, int npairs, const double* u,
+                           const double* e1, int nang, double* S11, double* S12, bdda_c64* S);
+
 /* ---------------------------------------------------------------- device
  * Device-resident variants (pointers from bdda_dev_alloc, complex<double>
  * layout) for callers that keep B and X in HBM. */
@@ -134,6 +168,11 @@ int bdda_solve_dev(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64
 /* device block matvec in the given precision; dX/dY hold complex<float>
  * (prec FP32) or complex<double> (prec FP64), ld = f*nocc */
 int bdda_mbt_apply_occupied_dev(bdda_mbt* h, int prec, int jacobi, const void* dX, void* dY, int ncols);
+/* bdda_cross_sections / bdda_scattering_matrix on device blocks (ld = f*nocc) */
+int bdda_cross_sections_dev(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX, int L, int alpha_mode, double* Cext,
+                            double* Cabs, double* Csca);
+int bdda_scattering_matrix_dev(bdda_mbt* h, const bdda_c64* dX, int npairs, const double* u, const double* e1,
+                               int nang, double* S11, double* S12, bdda_c64* S);
 /* per-pass kernel timing of the following applies (HIP events on the
  * handle's stream); pass_ms receives z-fwd, y-fwd, x-fused, y-inv, z-inv */
 int bdda_mbt_set_timing(bdda_mbt* h, int on);

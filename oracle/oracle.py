@@ -247,3 +247,70 @@ def dense_system(n, f, lf, k, address, diag):
     A = full[np.ix_(idx, idx)]
     A = A + np.diag(np.asarray(diag, dtype=np.complex128))
     return A
+
+
+# ---------------------------------------------------------------------------
+# observables (numpy restatements)
+# ---------------------------------------------------------------------------
+
+def cross_sections(B, X, diag, f, k, alpha_each=False):
+    """Cext, Cabs, Csca per column, restating Result_manager.cpp:65-122:
+    Cext = 4 pi k Im(EHinc.col(l).dot(PM.col(l))) (Eigen's dot conjugates its
+    left operand), Cabs = 4 pi k sum (Im(conj(1/alpha)) - 2k^3/3) |P|^2 with
+    alpha of dipole 0 for every dipole (alpha_each=False, as the reference)
+    or each dipole's own; 1/alpha = DIAG_A (Solver.cpp:31-47)."""
+    B = np.asarray(B, dtype=np.complex128).reshape(B.shape[0], -1)
+    X = np.asarray(X, dtype=np.complex128).reshape(X.shape[0], -1)
+    diag = np.asarray(diag, dtype=np.complex128)
+    rows = X.shape[0]
+    d = diag if alpha_each else np.tile(diag[:f], rows // f)
+    w = np.imag(np.conj(d)) - 2.0 * k ** 3 / 3.0
+    Cext = 4 * np.pi * k * np.imag(np.sum(np.conj(B) * X, axis=0))
+    Cabs = 4 * np.pi * k * np.sum(w[:, None] * np.abs(X) ** 2, axis=0)
+    return Cext, Cabs, Cext - Cabs
+
+
+def far_field(X, positions, f, k, ndirs):
+    """Far-field vector E' = k^2 sum_j [(I - n n) P_j - n x M_j] e^{-ik n.r_j}
+    (E_far = E' e^{ikr}/r) for each column of X at unit directions ndirs (m, 3).
+    Returns (m, 3, L)."""
+    X = np.asarray(X, dtype=np.complex128).reshape(X.shape[0], -1)
+    N = positions.shape[0]
+    Xr = X.reshape(N, f, -1)
+    ph = np.exp(-1j * k * (ndirs @ positions.T))               # (m, N)
+    AP = np.einsum("mj,jcl->mcl", ph, Xr[:, :3, :])            # (m, 3, L)
+    nn = ndirs[:, :, None]
+    E = AP - nn * np.sum(nn * AP, axis=1, keepdims=True)
+    if f == 6:
+        AM = np.einsum("mj,jcl->mcl", ph, Xr[:, 3:, :])
+        E = E - np.cross(np.broadcast_to(nn, AM.shape), AM, axis=1)
+    return k * k * E
+
+
+def amplitude_matrix(X, positions, f, k, u, e1, nang):
+    """S1..S4 (npairs, nang, 4) of column pairs (2p: pol e1_p, 2p+1: u_p x e1_p)
+    in the plane of u_p and e1_p: S2 = -ik theta.E'(e1), S4 = -ik phi.E'(e1),
+    S3 = -ik theta.E'(e2), S1 = -ik phi.E'(e2), theta_a = pi a/(nang-1)."""
+    X = np.asarray(X, dtype=np.complex128).reshape(X.shape[0], -1)
+    u = np.asarray(u, dtype=np.float64).reshape(-1, 3)
+    e1 = np.asarray(e1, dtype=np.float64).reshape(-1, 3)
+    th = np.pi * np.arange(nang) / (nang - 1)
+    out = np.zeros((u.shape[0], nang, 4), dtype=np.complex128)
+    for p in range(u.shape[0]):
+        n = np.cos(th)[:, None] * u[p] + np.sin(th)[:, None] * e1[p]
+        tv = -np.sin(th)[:, None] * u[p] + np.cos(th)[:, None] * e1[p]
+        pv = np.cross(u[p], e1[p])
+        E = far_field(X[:, 2 * p:2 * p + 2], positions, f, k, n)   # (nang, 3, 2)
+        eth = np.einsum("mc,mcl->ml", tv, E)
+        eph = np.einsum("c,mcl->ml", pv, E)
+        out[p, :, 0] = -1j * k * eph[:, 1]
+        out[p, :, 1] = -1j * k * eth[:, 0]
+        out[p, :, 2] = -1j * k * eth[:, 1]
+        out[p, :, 3] = -1j * k * eph[:, 0]
+    return out
+
+
+def scattering_matrix(S):
+    """S11, S12 from S1..S4 (Bohren & Huffman 3.16)."""
+    m2 = np.abs(S) ** 2
+    return 0.5 * m2.sum(axis=-1), 0.5 * (m2[..., 1] - m2[..., 0] + m2[..., 3] - m2[..., 2])

@@ -65,3 +65,15 @@ def test_fullsize_mixed_solve_matches_fp64(bdda, big):
     assert i64["converged"] == 1
     # two solutions at residual 1e-5 differ by at most cond(A) x 1e-5
     assert rel(Xm, X64) <= 1e-3
+    # cross sections of the mixed solve within 1e-4 of the FP64 solve's
+    xm = op.cross_sections(B, Xm)
+    x64 = op.cross_sections(B, X64)
+    print("Cext mixed", xm["Cext"][:4], "fp64", x64["Cext"][:4], flush=True)
+    for q in ("Cext", "Cabs", "Csca"):
+        assert np.abs(xm[q] - x64[q]).max() <= 1e-4 * np.abs(x64[q]).max(), q
+    w = W.waves
+    u, e1 = w.dirs[0::2], w.pols[0::2].real
+    S11m, S12m, _ = op.scattering_matrix(Xm, u, e1)
+    S11d, S12d, _ = op.scattering_matrix(X64, u, e1)
+    assert np.abs(S11m - S11d).max() <= 1e-4 * S11d.max()
+    assert np.abs(S12m - S12d).max() <= 1e-4 * S11d.max()
