@@ -168,6 +168,29 @@ def observables_dev(pkg, op, dB, dX, waves, s, nang=181):
     return {"Cext": out[0], "Cabs": out[1], "Csca": out[2], "S11": S11, "S12": S12}
 
 
+def serving_leg(pkg, nreq, rate):
+    """Poisson stream of solve requests through the serving loop (mixed
+    precision, block COCG-rq-Jacobi, tol 1e-5): p50/p99 latency, solves/s."""
+    S = importlib.import_module("block_dda_amd.serving")
+    T = importlib.import_module("block_dda_amd.targets")
+    t0 = time.time()
+    scenes = S.synthetic_scenes(T, 8, seed=2026, nmin=64, nmax=8192, across=(4, 8), max_grid=128)
+    reqs = S.synthetic_requests(T, scenes, nreq, rate=rate, seed=2027, s_choices=(2, 8, 32, 64), tol=1e-5)
+    log("serving: %d scenes (nocc %s) built in %.1fs" % (len(scenes), [sc.meta["nocc"] for sc in scenes],
+                                                           time.time() - t0))
+    budget = 200 << 30
+    sv = S.Server(S.GpuBackend(pkg), hbm_budget=budget, workers=4, max_cols=64).start()
+    S.replay(sv, reqs)
+    sv.drain()
+    st = sv.stats()
+    sv.stop()
+    st["rate_req_per_s"] = rate
+    st["scenes"] = [sc.meta for sc in scenes]
+    st["s_choices"] = [2, 8, 32, 64]
+    log("serving: %s" % {k: v for k, v in st.items() if k != "scenes"})
+    return st
+
+
 def compare_observables(mx, ref, geom_cs):
     """Errors of the mixed-precision observables against the FP64 solve's."""
     res = {}
@@ -195,6 +218,9 @@ def main():
     ap.add_argument("--matvec-reps", type=int, default=5)
     ap.add_argument("--matvec-only", action="store_true",
                     help="time the block matvecs and exit (for counter runs)")
+    ap.add_argument("--serving-requests", type=int, default=40,
+                    help="requests in the serving stream leg (0 skips it)")
+    ap.add_argument("--serving-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes: the known-size "
                          "reference for HBM byte counters")
@@ -286,6 +312,7 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "kernel": "k_xfused<float,3> (x FFT fwd + G^ multiply + x FFT inv)",
                 "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms}
+    serving = serving_leg(pkg, args.serving_requests, args.serving_rate) if args.serving_requests > 0 else None
     cpu = None
     if not args.no_cpu:
         try:
@@ -306,6 +333,7 @@ def main():
         if fp64 else None,
         "speedup_mixed_vs_fp64": (fp64["wall_s"] / tts) if fp64 else None,
         "observables": obs,
+        "serving": serving,
         "init_s": init_s,
         "matvec": {kk: {"pass_ms": v["pass_ms"], "matvec_GBs": v["matvec_GBs"],
                         "pass_GBs": v["pass_GBs"]} for kk, v in mv.items()},

@@ -99,9 +99,21 @@ SIGNATURES = {
     "bdda_scattering_matrix": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp,
                                               _vp]),
     "bdda_scattering_matrix_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _vp]),
+    "bdda_estimate_bytes": (ctypes.c_int, [_ip, ctypes.c_int, _i64, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                           ctypes.POINTER(_i64)]),
+    "bdda_mbt_set_workspace_limit": (ctypes.c_int, [_vp, ctypes.c_size_t]),
 }
 
 ALPHA_FIRST, ALPHA_EACH = 0, 1
+
+
+def estimate_bytes(n, f, nocc, L, mode=MODE_MIXED):
+    """HBM estimate in bytes: dict(init_peak, resident, workspace, krylov)."""
+    vals = [_i64() for _ in range(4)]
+    keep, p = _n3(n)
+    _check(lib().bdda_estimate_bytes(p, int(f), int(nocc), int(L), int(mode), *[ctypes.byref(v) for v in vals]))
+    return dict(zip(("init_peak", "resident", "workspace", "krylov"), (v.value for v in vals)))
 
 
 def lib():
@@ -252,6 +264,9 @@ class MbtOperator:
             _check(rc)
         return X, info.as_dict(), hist[:hist_len]
 
+
+    def set_workspace_limit(self, nbytes):
+        _check(lib().bdda_mbt_set_workspace_limit(self.h, int(nbytes)))
 
     def cross_sections(self, B, X, alpha_mode=ALPHA_FIRST):
         """Cext, Cabs, Csca per column (Result_manager.cpp:65-122), FP64 on the GPU."""

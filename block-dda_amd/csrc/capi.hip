@@ -33,9 +33,9 @@ struct bdda_mbt {
     }
 };
 
-#define BDDA_TRY(body)                                                              \
+#define BDDA_TRY(...)                                                               \
     try {                                                                           \
-        body                                                                        \
+        __VA_ARGS__                                                                 \
     } catch (const HipError& e) {                                                   \
         set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " at " + e.where); \
         return (e.code == hipErrorOutOfMemory) ? BDDA_ENOMEM : BDDA_EHIP;           \
@@ -473,6 +473,40 @@ int bdda_scattering_matrix_dev(bdda_mbt* h, const bdda_c64* dX, int npairs, cons
 {
     if (!h || !dX) return fail(BDDA_EINVAL, "null pointer");
     BDDA_TRY({ return sm_dev(h, reinterpret_cast<const c64*>(dX), npairs, u, e1, nang, S11, S12, S); })
+}
+
+int bdda_estimate_bytes(const int n[3], int f, int64_t nocc, int L, int mode, int64_t* init_peak,
+                        int64_t* resident, int64_t* workspace, int64_t* krylov)
+{
+    if (!n || !init_peak || !resident || !workspace || !krylov) return fail(BDDA_EINVAL, "null pointer");
+    if (f != 3 && f != 6) return fail(BDDA_EINVAL, "unknown f value (3 or 6)");
+    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || nocc < 1 || L < 1) return fail(BDDA_EINVAL, "bad sizes");
+    if (mode != BDDA_MODE_FP64 && mode != BDDA_MODE_FP32 && mode != BDDA_MODE_MIXED) return fail(BDDA_EINVAL, "bad mode");
+    BDDA_TRY({
+        MbtDims d{n[0], n[1], n[2], f, choose_padded(n[0], 8, f == 3 ? 1 : 8), choose_padded(n[1], 1, 1),
+                  choose_padded(n[2], 1, 1)};
+        const int Pzs = (int)ceil_div(d.Pz, 16) * 16;
+        const int64_t nc = (f == 3) ? 6 : 9;
+        const int64_t G = (int64_t)d.Px * d.Py * d.Pz;
+        const int64_t gh = (int64_t)(d.Px / 2 + 1) * (d.Py / 2 + 1) * (d.Pz / 2 + 1) * nc * (16 + 8);
+        const int64_t C = (int64_t)n[0] * n[1] * n[2];
+        const int64_t rows = (int64_t)f * nocc;
+        *resident = gh + 4 * C + 4 * nocc + rows * (16 + 16 + 8 + 8) + (int64_t)(d.Px + d.Py + d.Pz) * 24;
+        *init_peak = *resident + 2 * G * nc * 16;
+        const int64_t ws = (int64_t)mbt_workspace_per_column(d, Pzs, mode == BDDA_MODE_FP32 ? 8 : 16) * L;
+        const int64_t kry = rows * L * (mode == BDDA_MODE_FP64 ? 10 * 16 : mode == BDDA_MODE_MIXED ? 4 * 16 + 10 * 8
+                                                                                                 : 2 * 16 + 10 * 8);
+        *workspace = ws;
+        *krylov = kry;
+        return BDDA_OK;
+    })
+}
+
+int bdda_mbt_set_workspace_limit(bdda_mbt* h, size_t bytes)
+{
+    if (!h) return fail(BDDA_EINVAL, "null pointer");
+    h->op->ws_limit = bytes;
+    return BDDA_OK;
 }
 
 int bdda_qr_reduced(const bdda_c64* A, int64_t m, int n, bdda_c64* Q, bdda_c64* R)

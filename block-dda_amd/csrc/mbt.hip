@@ -307,6 +307,11 @@ template void mbt_apply_occupied<float>(MbtOperator*, const c32*, int64_t, c32*,
 template void mbt_apply_occupied<double>(MbtOperator*, const c64*, int64_t, c64*, int64_t, int, const c64*,
                                          const c64*, int64_t);
 
+size_t mbt_workspace_per_column(const MbtDims& d, int Pzs, size_t elem_bytes)
+{
+    return elem_bytes * (size_t)d.f * ((size_t)d.nx * d.ny + (size_t)d.nx * d.Py) * (size_t)Pzs;
+}
+
 double mbt_algorithmic_bytes(const MbtOperator* op, int s, int64_t nocc, size_t c)
 {
     const MbtDims& d = op->d;

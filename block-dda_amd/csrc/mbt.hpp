@@ -54,6 +54,7 @@ struct MbtOperator {
     // workspace for the passes (grown on demand)
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    size_t ws_limit = 0;        // > 0: block matvecs run in column chunks within it
     // occupancy map for the occupied-vector path
     int* c2o = nullptr;         // cell -> occupied index or -1
     int64_t c2o_cells = 0;
@@ -94,6 +95,9 @@ extern template void mbt_apply_precision<float>(MbtOperator*, const cplx<float>*
                                                 int, const cplx<float>*, const cplx<float>*, const int*);
 extern template void mbt_apply_precision<double>(MbtOperator*, const cplx<double>*, int64_t, cplx<double>*, int64_t,
                                                  int, const cplx<double>*, const cplx<double>*, const int*);
+
+// pass workspace of one column (U and V grids) in precision R
+size_t mbt_workspace_per_column(const MbtDims& d, int Pzs, size_t elem_bytes);
 
 // bytes per block matvec under the minimal fused pipeline (the algorithmic
 // model used for roofline numbers)

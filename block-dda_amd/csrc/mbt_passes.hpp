@@ -351,6 +351,7 @@ void apply_impl(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int6
     const int64_t nxny = (int64_t)d.nx * d.ny;
     const int64_t nU = (int64_t)s * F * nxny * Pzs;
     const int64_t nV = (int64_t)s * F * d.nx * d.Py * Pzs;
+    // (mbt_workspace_per_column is this per column)
     mbt_ensure_workspace(op, sizeof(cplx<R>) * (nU + nV));
     cplx<R>* U = (cplx<R>*)op->ws;
     cplx<R>* V = U + nU;
@@ -424,8 +425,15 @@ template <typename R>
 void mbt_apply_precision(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int64_t ldy, int s,
                          const cplx<R>* diag, const cplx<R>* jpre, const int* c2o)
 {
-    if (op->d.f == 3) apply_impl<R, 3>(op, X, ldx, Y, ldy, s, diag, jpre, c2o);
-    else apply_impl<R, 6>(op, X, ldx, Y, ldy, s, diag, jpre, c2o);
+    // columns in chunks whose pass workspace fits the operator's limit
+    const size_t per_col = mbt_workspace_per_column(op->d, op->Pzs, sizeof(cplx<R>));
+    int chunk = s;
+    if (op->ws_limit > 0) chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)s, op->ws_limit / per_col));
+    for (int j0 = 0; j0 < s; j0 += chunk) {
+        const int sc = std::min(chunk, s - j0);
+        if (op->d.f == 3) apply_impl<R, 3>(op, X + j0 * ldx, ldx, Y + j0 * ldy, ldy, sc, diag, jpre, c2o);
+        else apply_impl<R, 6>(op, X + j0 * ldx, ldx, Y + j0 * ldy, ldy, sc, diag, jpre, c2o);
+    }
 }
 
 }  // namespace bdda

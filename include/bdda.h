@@ -153,6 +153,21 @@ int bdda_cross_sections(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdda_
 int bdda_scattering_matrix(bdda_mbt* h, const bdda_c64* X, int64_t ldx, int npairs, const double* u,
                            const double* e1, int nang, double* S11, double* S12, bdda_c64* S);
 
+/* ---------------------------------------------------------------- memory
+ * No reference counterpart (the reference sizes nothing ahead of time).
+ * HBM estimate for a target of nocc dipoles on the n lattice solved with L
+ * right-hand sides in `mode`: init_peak while bdda_mbt_create runs (the full
+ * kernel grid is transformed in FP64 before its octant is kept), resident
+ * bytes of the operator with its target, the pass workspace of an L-column
+ * block matvec (allocated at the first solve and kept by the handle; see
+ * bdda_mbt_set_workspace_limit) and the Krylov blocks of one solve (freed
+ * when it returns). */
+int bdda_estimate_bytes(const int n[3], int f, int64_t nocc, int L, int mode, int64_t* init_peak,
+                        int64_t* resident, int64_t* workspace, int64_t* krylov);
+/* Cap the pass workspace of a handle: block matvecs whose workspace would
+ * exceed `bytes` run in column chunks (0 = no cap, the default). */
+int bdda_mbt_set_workspace_limit(bdda_mbt* h, size_t bytes);
+
 /* ---------------------------------------------------------------- device
  * Device-resident variants (pointers from bdda_dev_alloc, complex<double>
  * layout) for callers that keep B and X in HBM. */

@@ -1,0 +1,142 @@
+"""Scheduler of the serving loop (block-dda_amd/serving.py) on a CPU stand-in
+backend: every request completes with its own columns, same-scene requests
+merge, batches of different scenes overlap, the HBM budget holds at every
+admission, idle operators are evicted LRU-first, and a request too large
+for the budget fails alone without stopping the server."""
+import importlib
+import threading
+import time
+
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def S(bdda):
+    return importlib.import_module("block_dda_amd.serving")
+
+
+@pytest.fixture(scope="module")
+def T(bdda):
+    return importlib.import_module("block_dda_amd.targets")
+
+
+class FakeBackend:
+    """Sizes proportional to the scene; a solve sleeps and returns 2 B."""
+
+    def __init__(self, base=100, ws_per_col=5, kry_per_col=3, solve_s=0.02):
+        self.base, self.ws_per_col, self.kry_per_col, self.solve_s = base, ws_per_col, kry_per_col, solve_s
+        self.created = []
+        self.released = []
+        self.lock = threading.Lock()
+        self.live = 0
+        self.max_live = 0
+
+    def estimate(self, scene, cols):
+        b = self.base * scene.target.nocc // 10
+        return 2 * b, b, self.ws_per_col * cols, self.kry_per_col * cols
+
+    def create(self, scene):
+        time.sleep(0.005)
+        with self.lock:
+            self.created.append(scene.key)
+        return {"scene": scene}
+
+    def release(self, h):
+        with self.lock:
+            self.released.append(h["scene"].key)
+
+    def solve(self, h, B, tol):
+        with self.lock:
+            self.live += 1
+            self.max_live = max(self.max_live, self.live)
+        time.sleep(self.solve_s)
+        with self.lock:
+            self.live -= 1
+        return 2.0 * B, {"iters": 1, "converged": 1}
+
+    def observables(self, h, B, X):
+        return {"Cext": [float(np.abs(X).sum())]}
+
+
+def scenes(S, T, n):
+    out = []
+    for i in range(n):
+        t = T.Target((4, 4, 4), 0.5, np.arange(10 + i, dtype=np.int32))
+        out.append(S.Scene(t, 1.0 + 0.1 * i, np.ones(3 * t.nocc) * (5 + i)))
+    return out
+
+
+def req(S, sc, cols, seed):
+    rng = np.random.default_rng(seed)
+    return S.Request(sc, rng.standard_normal((sc.rows, cols)) + 0j)
+
+
+def test_all_complete_with_their_columns_and_merge(S, T):
+    be = FakeBackend()
+    sv = S.Server(be, hbm_budget=10 ** 6, workers=3).start()
+    sc = scenes(S, T, 3)
+    # hold the workers busy so that same-scene requests pile up and merge
+    reqs = [req(S, sc[i % 3], 2 + (i % 2) * 6, i) for i in range(12)]
+    for r in reqs:
+        sv.submit(r)
+    assert sv.drain(timeout=30)
+    st = sv.stats()
+    sv.stop()
+    assert st["requests"] == 12 and st["failed"] == 0
+    for r in reqs:
+        assert np.array_equal(r.X, 2.0 * r.B)
+        assert r.finish >= r.start >= r.arrival
+    assert st["batches"] < 12 and st["merged_requests"] > 0
+    assert st["max_concurrent_batches"] >= 2 and be.max_live >= 2
+    assert sorted(set(be.created)) == sorted(s.key for s in sc)   # one operator per scene
+
+
+def test_budget_and_lru_eviction(S, T):
+    be = FakeBackend(solve_s=0.01)
+    sc = scenes(S, T, 4)
+    # each operator: base = 100 .. 130 (+ build transient 2x), budget fits about two
+    sv = S.Server(be, hbm_budget=400, workers=2).start()
+    reqs = [req(S, sc[i % 4], 2, i) for i in range(16)]
+    for r in reqs:
+        sv.submit(r)
+        time.sleep(0.003)
+    assert sv.drain(timeout=30)
+    st = sv.stats()
+    sv.stop()
+    assert st["failed"] == 0
+    assert st["peak_bytes"] <= 400
+    assert st["evictions"] > 0 and len(be.released) >= st["evictions"]
+
+
+def test_too_large_request_fails_alone(S, T):
+    be = FakeBackend()
+    sc = scenes(S, T, 2)
+    sv = S.Server(be, hbm_budget=250, workers=2).start()
+    big = req(S, sc[1], 64, 0)            # 110 base + 64 * 8 > 250
+    small = req(S, sc[0], 2, 1)
+    sv.submit(big)
+    sv.submit(small)
+    assert sv.drain(timeout=30)
+    sv.stop()
+    assert big.error is not None and "budget" in big.error
+    assert small.error is None and np.array_equal(small.X, 2.0 * small.B)
+
+
+def test_no_merge_mode_and_stream_generator(S, T):
+    be = FakeBackend(solve_s=0.002)
+    sc = scenes(S, T, 2)
+    sv = S.Server(be, hbm_budget=10 ** 6, workers=1, merge=False).start()
+    reqs = [req(S, sc[0], 2, i) for i in range(5)]
+    for r in reqs:
+        sv.submit(r)
+    assert sv.drain(timeout=30)
+    st = sv.stats()
+    sv.stop()
+    assert st["batches"] == 5 and st["merged_requests"] == 0
+    # Poisson offsets increase, s from the choices, B = plane waves of the scene
+    rq = S.synthetic_requests(T, sc, 20, rate=4.0, seed=3, s_choices=(2, 8))
+    offs = [r.offset for r in rq]
+    assert all(b > a for a, b in zip(offs, offs[1:]))
+    assert {r.cols for r in rq} <= {2, 8}
+    assert abs(np.mean(np.diff([0.0] + offs)) - 0.25) < 0.15
