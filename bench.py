@@ -33,6 +33,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "time-to-solution & p99 latency at rel.res 1e-5; mixed-prec vs FP64 on MI355X"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the FP32 fused x pass, from FETCH_SIZE and
+# WRITE_SIZE counter runs scaled by a 2 GiB device copy
+# (profiles/mbt_matvec_agg16384_hbm_traffic.md); counters cannot be read
+# from inside this process, so the figure is carried here per workload.
+MEASURED_X_TRAFFIC = {"agg16384": 24.13e9}
 
 
 def load_pkg():
@@ -309,8 +314,8 @@ def main():
     dom_ms = xf["pass_ms"]["x_fused"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_xfused<float,3> (x FFT fwd + G^ multiply + x FFT inv)",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_X_TRAFFIC.get(args.config),
+                "kernel": "k_xfused_plan<float> at Px=%d (x FFT fwd + G^ multiply + x FFT inv)" % P[0],
                 "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms}
     serving = serving_leg(pkg, args.serving_requests, args.serving_rate) if args.serving_requests > 0 else None
     cpu = None
