@@ -73,7 +73,9 @@ __global__ __launch_bounds__(NT_RED) void k_gram_partial(const cplx<R>* __restri
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int ta = (a + 3) >> 2, tb = (b + 3) >> 2;
-    const int las = 4 * ta, lbs = 4 * tb;  // LDS row pitch (elements)
+    // LDS row pitch in elements: one more than the padded width, so that the
+    // column-wise tile stores of consecutive rows fall in different banks
+    const int las = 4 * ta + 1, lbs = 4 * tb + 1;
     cplx<R>* As = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* Bs = As + (size_t)TR * las;
     const int ntile = ta * tb;
@@ -146,17 +148,18 @@ __global__ __launch_bounds__(NT_RED) void k_gram_partial(const cplx<R>* __restri
         }
 }
 
-// out[p] = sum_k partial[k np + p]; one 256-thread group per 64 entries:
-// four k-slices per entry, combined in LDS in a fixed order (deterministic)
-__global__ __launch_bounds__(256) void k_reduce_partial(const c64* __restrict__ partial, int nblk, int np,
-                                                        c64* __restrict__ out)
+// out[p] = sum_k partial[k np + p]; one 1024-thread group per 64 entries:
+// sixteen k-slices per entry, combined in LDS in a fixed order (deterministic)
+constexpr int RED_SLICES = 16;
+__global__ __launch_bounds__(64 * RED_SLICES) void k_reduce_partial(const c64* __restrict__ partial, int nblk, int np,
+                                                                    c64* __restrict__ out)
 {
-    __shared__ double sh[2][4][64];
+    __shared__ double sh[2][RED_SLICES][64];
     const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
     const int p = blockIdx.x * 64 + lane;
     double sx = 0, sy = 0;
     if (p < np)
-        for (int k = slice; k < nblk; k += 4) {
+        for (int k = slice; k < nblk; k += RED_SLICES) {
             const c64 v = partial[(int64_t)k * np + p];
             sx += v.x;
             sy += v.y;
@@ -164,9 +167,15 @@ __global__ __launch_bounds__(256) void k_reduce_partial(const c64* __restrict__ 
     sh[0][slice][lane] = sx;
     sh[1][slice][lane] = sy;
     __syncthreads();
-    if (slice == 0 && p < np)
-        out[p] = mk(((sh[0][0][lane] + sh[0][1][lane]) + sh[0][2][lane]) + sh[0][3][lane],
-                    ((sh[1][0][lane] + sh[1][1][lane]) + sh[1][2][lane]) + sh[1][3][lane]);
+    if (slice == 0 && p < np) {
+        double tx = 0, ty = 0;
+#pragma unroll
+        for (int k = 0; k < RED_SLICES; ++k) {
+            tx += sh[0][k][lane];
+            ty += sh[1][k][lane];
+        }
+        out[p] = mk(tx, ty);
+    }
 }
 
 template <typename R, bool CONJ>
@@ -321,7 +330,7 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
     const int ntile = ((a + 3) / 4) * ((b + 3) / 4);
     if (a < 1 || b < 1 || np > 4096 || ntile > NT_RED) throw HipError{hipErrorInvalidValue, "gram: block too wide"};
     const size_t c = sizeof(cplx<R>);
-    const int pitch = 4 * ((a + 3) / 4) + 4 * ((b + 3) / 4);
+    const int pitch = 4 * ((a + 3) / 4) + 4 * ((b + 3) / 4) + 2;
     int TR = (int)std::min<int64_t>(128, 32768 / (pitch * c));
     TR = std::max(TR, 1);
     int64_t rpb = std::max<int64_t>(TR, ceil_div(std::max<int64_t>(rows, 1), MAX_PARTIAL_BLOCKS));
@@ -336,7 +345,7 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
         hipLaunchKernelGGL((k_gram_partial<R, false>), dim3(nblk), dim3(NT_RED), shm, w.stream, A, lda, a, B, ldb, b,
                            rows, TR, rpb, w.partial);
     BDDA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, 64)), dim3(256), 0, w.stream,
+    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, 64)), dim3(64 * RED_SLICES), 0, w.stream,
                        (const c64*)w.partial, nblk, np, w.red);
     BDDA_HIP(hipGetLastError());
     HostMat out(np);
@@ -359,7 +368,8 @@ c64 dot_all(BlockWork& w, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64
         hipLaunchKernelGGL((k_dot_partial<R, false>), dim3(nblk), dim3(NT_RED), 0, w.stream, A, lda, B, ldb, rows, s,
                            w.partial);
     BDDA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_partial, dim3(1), dim3(256), 0, w.stream, (const c64*)w.partial, nblk, 1, w.red);
+    hipLaunchKernelGGL(k_reduce_partial, dim3(1), dim3(64 * RED_SLICES), 0, w.stream, (const c64*)w.partial, nblk, 1,
+                       w.red);
     BDDA_HIP(hipGetLastError());
     BDDA_HIP(hipMemcpyAsync(w.host_pinned + w.host_elems, w.red, sizeof(c64), hipMemcpyDeviceToHost, w.stream));
     BDDA_HIP(hipStreamSynchronize(w.stream));

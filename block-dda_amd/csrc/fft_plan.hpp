@@ -83,8 +83,11 @@ struct PlanFFT {
 
     __device__ __forceinline__ PlanFFT(const cplx<R>* tw_, cplx<R>* lds_) : tw(tw_), lds(lds_) {}
 
-    // stage-0 inputs: ld(p, q, c)
-    template <bool INV, class LD>
+    // stage-0 inputs: ld(p, q, c).  HALF: the input is a zero-padded line of
+    // at most (P+1)/2 values (P >= 2n - 1), so butterfly inputs at
+    // p = j + r P/RAD with r P/RAD >= (P+1)/2 are zero whatever j is; they are
+    // set at compile time and the arithmetic on them folds away.
+    template <bool INV, bool HALF = false, class LD>
     __device__ __forceinline__ void load(LD&& ld)
     {
         constexpr int RAD = rad<0, INV>(), nb = P / RAD, total = nb * NQ;
@@ -93,16 +96,22 @@ struct PlanFFT {
             const int b = threadIdx.x + i * NT;
             if (total % NT == 0 || b < total) {
                 const int q = b & (NQ - 1), j = b >> QS;
+                static_for<0, RAD>([&](auto RR) {
+                    constexpr int r = decltype(RR)::value;
 #pragma unroll
-                for (int r = 0; r < RAD; ++r)
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) v[c][i * RAD + r] = ld(j + r * nb, q, c);
+                    for (int c = 0; c < NC; ++c) {
+                        if constexpr (HALF && r * nb >= (P + 1) / 2) v[c][i * RAD + r] = czero<R>();
+                        else v[c][i * RAD + r] = ld(j + r * nb, q, c);
+                    }
+                });
             }
         });
     }
 
-    // last-stage outputs: st(p, q, c, value)
-    template <bool INV, class ST>
+    // last-stage outputs: st(p, q, c, value).  HALF: only outputs p < (P+1)/2
+    // are wanted (the inverse transform keeps n <= (P+1)/2 of them); the
+    // others are never stored and their arithmetic is dead code.
+    template <bool INV, bool HALF = false, class ST>
     __device__ __forceinline__ void store(ST&& st)
     {
         constexpr int S = NS - 1, RAD = rad<S, INV>(), nb = P / RAD, total = nb * NQ;
@@ -111,10 +120,13 @@ struct PlanFFT {
             const int b = threadIdx.x + i * NT;
             if (total % NT == 0 || b < total) {
                 const int q = b & (NQ - 1), j = b >> QS;
+                static_for<0, RAD>([&](auto RR) {
+                    constexpr int r = decltype(RR)::value;
+                    if constexpr (!(HALF && r * nb >= (P + 1) / 2)) {
 #pragma unroll
-                for (int r = 0; r < RAD; ++r)
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) st(j + r * nb, q, c, v[c][i * RAD + r]);
+                        for (int c = 0; c < NC; ++c) st(j + r * nb, q, c, v[c][i * RAD + r]);
+                    }
+                });
             }
         });
     }

@@ -135,12 +135,12 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
     const cplx<R>* src = in + (outer * nlin) * pitch + z0;
     cplx<R>* dst = out + (outer * nlout) * pitch + z0;
     PlanFFT<R, PL, NT, NQ, 1, 1> F(tw, lds);
-    F.template load<INV>([&](int p, int q, int) {
+    F.template load<INV, !INV>([&](int p, int q, int) {
         return (p < nlin && q < tz) ? src[(int64_t)p * pitch + q] : czero<R>();
     });
     __syncthreads();
     F.template run<INV>();
-    F.template store<INV>([&](int p, int q, int, cplx<R> v) {
+    F.template store<INV, INV>([&](int p, int q, int, cplx<R> v) {
         if (p < nlout && q < tz) dst[(int64_t)p * pitch + q] = v;
     });
 }
@@ -168,35 +168,44 @@ __global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, con
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     cplx<R>* base = V + (int64_t)rhs * F * cstride + (int64_t)ky * Pzs + kz0;
     PlanFFT<R, PL, NT, NQ, F, 1> X(tw, lds);
-    X.template load<false>([&](int p, int q, int c) {
+    X.template load<false, true>([&](int p, int q, int c) {
         return (p < d.nx && q < tqn) ? base[c * cstride + p * plane + q] : czero<R>();
     });
     __syncthreads();
     X.template run<false>();
+    // G^ is stored for one octant; the (kx, ky, kz) image of a reduced element
+    // carries the signs of the axis reflections: G'_ab = s_a s_b G_ab for the
+    // symmetric part, and det(s) s_a s_b A_ab for the antisymmetric part of f = 6
     const int Hy = d.Py / 2 + 1, Hz = d.Pz / 2 + 1;
-    int flyz = 0, kyr = ky;
-    if (ky > d.Py / 2) { kyr = d.Py - ky; flyz |= 2; }
+    static_assert(NT % NQ == 0, "a thread keeps one kz for all its butterflies");
+    const bool fy = ky > d.Py / 2;
+    const int kyr = fy ? d.Py - ky : ky;
+    const int kz = kz0 + (int)(threadIdx.x & (NQ - 1));
+    const bool fz = kz > d.Pz / 2;
+    const int kzr = fz ? d.Pz - kz : kz;
+    const int off_yz = (kyr * Hz + kzr) * NC;  // < 2^31 for every planned size
+    const int stride_x = Hy * Hz * NC;
+    const R sy = fy ? R(-1) : R(1), sz = fz ? R(-1) : R(1), syz = sy * sz;
     X.visit([&](int kx, int q, cplx<R>* h) {
         if (q >= tqn) return;
-        const int kz = kz0 + q;
-        int fl = flyz, kzr = kz, kxr = kx;
-        if (kz > d.Pz / 2) { kzr = d.Pz - kz; fl |= 4; }
-        if (kx > P / 2) { kxr = P - kx; fl |= 1; }
-        const cplx<R>* gp = Gh + (((int64_t)kxr * Hy + kyr) * Hz + kzr) * NC;
+        const bool fx = kx > P / 2;
+        const int kxr = fx ? P - kx : kx;
+        const cplx<R>* gp = Gh + (kxr * stride_x + off_yz);
+        const R sx = fx ? R(-1) : R(1), sxy = sx * sy, sxz = sx * sz;
         cplx<R> gv[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            cplx<R> v = gp[c];
-            if (__popc(comp_odd_mask(c) & fl) & 1) v = -v;
-            gv[c] = v;
-        }
+        for (int c = 0; c < NC; ++c) gv[c] = gp[c];
+        gv[3] = sxy * gv[3];
+        gv[4] = sxz * gv[4];
+        gv[5] = syz * gv[5];
         const cplx<R> ex = h[0], ey = h[1], ez = h[2];
         cplx<R> ox = gv[0] * ex + gv[3] * ey + gv[4] * ez;
         cplx<R> oy = gv[3] * ex + gv[1] * ey + gv[5] * ez;
         cplx<R> oz = gv[4] * ex + gv[5] * ey + gv[2] * ez;
         if constexpr (F == 6) {
+            const R sxyz = sxy * sz;  // det of the reflection
             const cplx<R> hx = h[3], hy = h[4], hz = h[5];
-            const cplx<R> a = gv[6], bb = gv[7], cc = gv[8];
+            const cplx<R> a = (sxyz * sxy) * gv[6], bb = (sxyz * sxz) * gv[7], cc = (sxyz * syz) * gv[8];
             ox = ox + a * hy + bb * hz;
             oy = oy - a * hx + cc * hz;
             oz = oz - bb * hx - cc * hy;
@@ -209,7 +218,7 @@ __global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, con
         h[2] = oz;
     });
     X.template run<true>();
-    X.template store<true>([&](int p, int q, int c, cplx<R> v) {
+    X.template store<true, true>([&](int p, int q, int c, cplx<R> v) {
         if (p < d.nx && q < tqn) base[c * cstride + p * plane + q] = v;
     });
 }
@@ -237,8 +246,10 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
     const int nl = (int)min((int64_t)TL, nxny - l0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     const cplx<R>* Xr = X + rhs * ldx;
-    // gather of the occupied values into LDS [(p TL + line) F + c], one
-    // occupancy lookup per cell
+    // gather of the occupied values into LDS [(line F + c) PZ + p], one
+    // occupancy lookup per cell; rows of odd pitch PZ >= nz keep both the
+    // z-contiguous gather and the line-strided FFT loads free of bank conflicts
+    const int PZ = d.nz | 1;
     for (int e = threadIdx.x; e < TL * d.nz; e += NT) {
         const int p = e % d.nz, tl = e / d.nz;
         int64_t m = -1;
@@ -247,11 +258,11 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
             m = c2o ? (int64_t)c2o[cell] : cell;
         }
 #pragma unroll
-        for (int c = 0; c < F; ++c) lds[(p * TL + tl) * F + c] = (m >= 0) ? Xr[m * F + c] : czero<R>();
+        for (int c = 0; c < F; ++c) lds[(tl * F + c) * PZ + p] = (m >= 0) ? Xr[m * F + c] : czero<R>();
     }
     __syncthreads();
     PlanFFT<R, PL, NT, TL, F, F> Z(tw, lds);
-    Z.template load<false>([&](int p, int q, int c) { return p < d.nz ? lds[(p * TL + q) * F + c] : czero<R>(); });
+    Z.template load<false, true>([&](int p, int q, int c) { return p < d.nz ? lds[(q * F + c) * PZ + p] : czero<R>(); });
     __syncthreads();
     Z.template run<false>();
     Z.template store<false>([&](int p, int q, int c, cplx<R> v) { lds[(c * TL + q) * P + p] = v; });
@@ -296,8 +307,10 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
     Z.template load<true>([&](int p, int q, int c) { return lds[(c * TL + q) * P + p]; });
     __syncthreads();
     Z.template run<true>();
-    Z.template store<true>([&](int p, int q, int c, cplx<R> v) {
-        if (p < d.nz) lds[(q * d.nz + p) * F + c] = v;
+    // inverse output into LDS [(c TL + line) PZ + p], odd pitch PZ >= nz
+    const int PZ = d.nz | 1;
+    Z.template store<true, true>([&](int p, int q, int c, cplx<R> v) {
+        if (p < d.nz) lds[(c * TL + q) * PZ + p] = v;
     });
     __syncthreads();
     const cplx<R>* Xr = X + rhs * ldx;
@@ -307,10 +320,11 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
         const int64_t cell = l0 * d.nz + e;
         const int64_t m = c2o ? (int64_t)c2o[cell] : cell;
         if (m < 0) continue;
+        const int tl = e / d.nz, p = e - tl * d.nz;
 #pragma unroll
         for (int c = 0; c < F; ++c) {
             const int64_t i = m * F + c;
-            cplx<R> v = lds[e * F + c];
+            cplx<R> v = lds[(c * TL + tl) * PZ + p];
             if (diag) v = diag[i] * Xr[i] + v;
             if (jpre) v = v * jpre[i];
             Yr[i] = v;
