@@ -537,6 +537,59 @@ int qr_cgs2(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq
     return BDDA_OK;
 }
 
+// qr_reduced.cpp:9-31 step by step: modified Gram-Schmidt (each projection
+// uses the column as already updated), run twice per column; R(i,k) keeps
+// the first pass's coefficients, R(k,k) the norm after the second.  One dot
+// and one update per projection, so O(s^2) launches: for the standalone
+// export, not for the solver loops.
+template <typename R>
+int qr_mgs2(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s, HostMat& Rm)
+{
+    Rm.assign((size_t)s * s, mk(0.0, 0.0));
+    for (int k = 0; k < s; ++k) {
+        cplx<R>* qk = Q + k * ldq;
+        if ((const void*)A != (const void*)Q || lda != ldq) {
+            Combo<R> cp;
+            cp.nv = 1;
+            cp.V[0] = A + k * lda;
+            cp.ldv[0] = lda;
+            cp.a[0] = mk(1.0, 0.0);
+            combo<R>(w, cp, qk, ldq, rows, 1);
+        }
+        // the reference divides by whatever norm is left (qr_reduced.cpp:
+        // 28); a column whose norm fell to rounding level of its original is
+        // numerically dependent and reported as a breakdown instead
+        const double dep = (sizeof(R) == 8) ? 1e-12 : 1e-5;
+        const double orig = norm_fro<R>(w, qk, ldq, rows, 1);
+        double tt = 0.0;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int i = 0; i < k; ++i) {
+                const c64 sc = dot_all<R>(w, Q + i * ldq, ldq, qk, ldq, rows, 1, 1);
+                if (tt == 0.0) Rm[i + (size_t)k * s] = sc;
+                Combo<R> up;
+                up.nv = 2;
+                up.V[0] = qk;
+                up.ldv[0] = ldq;
+                up.a[0] = mk(1.0, 0.0);
+                up.V[1] = Q + i * ldq;
+                up.ldv[1] = ldq;
+                up.a[1] = mk(-sc.x, -sc.y);
+                combo<R>(w, up, qk, ldq, rows, 1);
+            }
+            tt = norm_fro<R>(w, qk, ldq, rows, 1);
+        }
+        if (!(tt > dep * orig) || !std::isfinite(tt)) return BDDA_EBREAKDOWN;
+        Rm[k + (size_t)k * s] = mk(tt, 0.0);
+        Combo<R> sc;
+        sc.nv = 1;
+        sc.V[0] = qk;
+        sc.ldv[0] = ldq;
+        sc.a[0] = mk(1.0 / tt, 0.0);
+        combo<R>(w, sc, qk, ldq, rows, 1);
+    }
+    return BDDA_OK;
+}
+
 // upper Cholesky of a Hermitian positive definite G = U^H U; false on failure
 static bool chol_upper(const HostMat& G, int s, HostMat& U, double rel_floor)
 {
@@ -726,6 +779,7 @@ bool hm_lu_solve(const HostMat& A_, const HostMat& B, int s, int nb, int full, H
                                 int64_t, int);                                                                 \
     template void fill_zero<R>(hipStream_t, cplx<R>*, int64_t, int64_t, int);                                  \
     template int qr_cgs2<R>(BlockWork&, const cplx<R>*, int64_t, cplx<R>*, int64_t, int64_t, int, HostMat&);   \
+    template int qr_mgs2<R>(BlockWork&, const cplx<R>*, int64_t, cplx<R>*, int64_t, int64_t, int, HostMat&);   \
     template int qr_fast<R>(BlockWork&, const cplx<R>*, int64_t, cplx<R>*, int64_t, int64_t, int, HostMat&, \
                             cplx<R>*);
 BDDA_INST(float)

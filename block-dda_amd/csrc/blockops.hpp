@@ -72,11 +72,19 @@ void accumulate_32_to_64(hipStream_t st, const c32* A, int64_t lda, c64* Z, int6
 template <typename R>
 void fill_zero(hipStream_t st, cplx<R>* Z, int64_t ldz, int64_t rows, int s);
 
-// reduced QR with first-pass R semantics (qr_reduced.cpp:9-31), column-
-// sequential classical Gram-Schmidt with one re-orthogonalisation pass.
-// Q may alias A.  Returns BDDA_OK or BDDA_EBREAKDOWN (a zero column).
+// reduced QR by column-sequential classical Gram-Schmidt with one
+// re-orthogonalisation pass (a whole column's projections per Gram launch);
+// the solvers' fallback when Cholesky QR fails.  Q may alias A.  Returns
+// BDDA_OK or BDDA_EBREAKDOWN (a numerically dependent column).
 template <typename R>
 int qr_cgs2(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s,
+            HostMat& Rm);
+
+// qr_reduced exactly as the reference orders it: modified Gram-Schmidt with
+// one re-orthogonalisation pass, R(i,k) from the first pass
+// (qr_reduced.cpp:9-31).  O(s^2) launches.  Q may alias A.
+template <typename R>
+int qr_mgs2(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s,
             HostMat& Rm);
 
 // reduced QR by Cholesky QR applied twice (two Gram + two update sweeps);

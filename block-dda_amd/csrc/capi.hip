@@ -528,7 +528,7 @@ int bdda_qr_reduced(const bdda_c64* A, int64_t m, int n, bdda_c64* Q, bdda_c64* 
         } fa{dA};
         BDDA_HIP(hipMemcpy(dA, A, sizeof(c64) * m * n, hipMemcpyHostToDevice));
         HostMat Rm;
-        int rc = qr_cgs2<double>(w, dA, m, dA, m, m, n, Rm);
+        int rc = qr_mgs2<double>(w, dA, m, dA, m, m, n, Rm);
         if (rc) return fail(rc, "qr_reduced: rank-deficient input (zero column after orthogonalisation)");
         BDDA_HIP(hipMemcpy(Q, dA, sizeof(c64) * m * n, hipMemcpyDeviceToHost));
         std::memcpy(R, Rm.data(), sizeof(c64) * n * n);

@@ -15,6 +15,10 @@ measured configurations:
              (s = 16)
   agg256     256-monomer aggregate with the agg16384 material and waves (a
              small case for tests)
+  agg4096    4096-monomer aggregate, 6 dipoles across each monomer, for the
+             wavelength sweep: `sweep(wl)` gives the workload at wavelength wl
+             (300-1000 nm) with m linear in wl from 2.0 + 1.0i to 2.4 + 0.5i and
+             4 directions x 2 polarisations (s = 8)
 
 Lengths are in nm for the aggregates and in units of the sphere radius for
 sphere16.  The polarizability is the lattice dispersion relation one.
@@ -32,6 +36,8 @@ CONFIGS = {
                    seed=256),
     "agg1024": dict(kind="agg", N=1024, Df=2.8, kf=1.0, a0=20.0, across=8, m=1.95 + 0.79j, wl=550.0, ndir=16,
                     seed=1024),
+    "agg4096": dict(kind="agg", N=4096, Df=2.8, kf=1.0, a0=20.0, across=6, m=2.2 + 0.75j, wl=650.0, ndir=4,
+                    seed=4096),
     "agg16384": dict(kind="agg", N=16384, Df=2.8, kf=1.0, a0=20.0, across=4, m=2.3 + 0.6j, wl=400.0, ndir=8,
                      seed=16384),
 }
@@ -58,8 +64,20 @@ class Workload:
                                        t.nocc, c["m"], c["wl"], self.s))
 
 
-def build(name):
-    cfg = CONFIGS[name]
+def sweep_material(wl):
+    """m(lambda) of the sweep: linear from 2.0 + 1.0i at 300 nm to 2.4 + 0.5i at 1000 nm."""
+    t = (wl - 300.0) / 700.0
+    return complex(2.0 + 0.4 * t, 1.0 - 0.5 * t)
+
+
+def sweep(wl, target=None):
+    """The agg4096 workload at wavelength wl (nm); pass `target` to reuse a built lattice."""
+    cfg = dict(CONFIGS["agg4096"], wl=float(wl), m=sweep_material(wl))
+    return build("agg4096", cfg=cfg, target=target)
+
+
+def build(name, cfg=None, target=None):
+    cfg = CONFIGS[name] if cfg is None else cfg
     t0 = time.time()
     if cfg["kind"] == "sphere":
         radius = 1.0
@@ -67,7 +85,8 @@ def build(name):
         k = cfg["ka"] / radius
         waves = T.plane_waves_xy_z(k)
     else:
-        tgt = T.aggregate_target(cfg["N"], cfg["Df"], cfg["kf"], cfg["a0"], cfg["across"], cfg["seed"])
+        tgt = target if target is not None else T.aggregate_target(cfg["N"], cfg["Df"], cfg["kf"], cfg["a0"],
+                                                                   cfg["across"], cfg["seed"])
         k = 2.0 * math.pi / cfg["wl"]
         waves = T.plane_waves_sphere(k, cfg["ndir"], cfg["seed"])
     alpha = T.alpha_ldr(cfg["m"], tgt.lf, k, S=0.0)

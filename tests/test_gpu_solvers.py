@@ -98,6 +98,15 @@ def test_qr_reduced_gpu(bdda, orc):
     Qo, Ro = orc.qr_reduced(A)
     assert np.abs(Q - Qo).max() < 1e-12 and np.abs(R - Ro).max() < 1e-12 * np.abs(Ro).max()
     assert np.abs(np.tril(R, -1)).max() == 0 and (np.diag(R).real > 0).all()
+    # nearly dependent columns (condition ~1e7): the modified Gram-Schmidt
+    # order of qr_reduced.cpp matters here, and the oracle follows it
+    base = rng.standard_normal(500) + 1j * rng.standard_normal(500)
+    An = np.stack([base + 1e-7 * (rng.standard_normal(500) + 1j * rng.standard_normal(500)) for _ in range(6)], axis=1)
+    Q, R = bdda.qr_reduced(An)
+    Qo, Ro = orc.qr_reduced(An)
+    assert np.abs(R - Ro).max() < 1e-12 * np.abs(Ro).max()
+    assert np.abs(Q.conj().T @ Q - np.eye(6)).max() < 1e-12
+    assert np.linalg.norm(An - Q @ R) < 1e-14 * np.linalg.norm(An)
     A[:, 3] = A[:, 1]
     with pytest.raises(bdda.BddaError) as e:
         bdda.qr_reduced(A)

@@ -238,3 +238,25 @@ def test_qr_reduced_invariants(orc):
     ph = np.diag(Rn) / np.abs(np.diag(Rn))
     assert np.allclose(Qn * ph[None, :], Q, atol=1e-13)
     assert np.allclose(Rn / ph[:, None], R, atol=1e-12)
+
+
+def test_qr_reduced_nearly_dependent_columns(orc):
+    """qr_reduced keeps only the first-pass projection coefficients in R
+    (the comma expressions at qr_reduced.cpp:18,22 make the `tt == 0` test
+    true only in pass 0).  With modified Gram-Schmidt the second-pass
+    coefficients it drops are O(u |a_k|) whatever the conditioning, so the
+    quirk moves results at rounding level only: on columns equal up to 1e-7
+    (condition number ~1e7) Q stays orthonormal after the second pass, Q R
+    reproduces A to rounding and R matches LAPACK's R (phases normalised)."""
+    rng = np.random.default_rng(12)
+    m, n = 80, 5
+    base = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    A = np.stack([base + 1e-7 * (rng.standard_normal(m) + 1j * rng.standard_normal(m)) for _ in range(n)], axis=1)
+    Q, R = orc.qr_reduced(A)
+    assert np.abs(Q.conj().T @ Q - np.eye(n)).max() < 1e-12
+    assert np.linalg.norm(A - Q @ R) <= 1e-14 * np.linalg.norm(A)
+    d = np.diag(R)
+    assert (d.real > 0).all() and not d.imag.any() and np.abs(np.tril(R, -1)).max() == 0
+    Qn, Rn = np.linalg.qr(A)
+    ph = np.diag(Rn) / np.abs(np.diag(Rn))
+    assert np.abs(Rn / ph[:, None] - R).max() <= 1e-12 * np.abs(R).max()
