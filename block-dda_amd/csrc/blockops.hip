@@ -11,6 +11,7 @@
 #include "blockops.hpp"
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace bdda {
@@ -146,6 +147,125 @@ __global__ __launch_bounds__(NT_RED) void k_gram_partial(const cplx<R>* __restri
             }
             __syncthreads();
         }
+}
+
+// ------------------------------------------------------------- MFMA Gram
+// G = op(A)^T B for a, b <= 16 on the matrix cores.  The complex product is
+// four real 16x16x4 MFMAs per four rows:
+//   Re += Ar^T Br -/+ Ai^T Bi,   Im += Ar^T Bi +/- Ai^T Br   (lower sign: op = adjoint)
+// with A^T's 16 x 4 operand taken straight from the LDS row tile (lane l
+// holds row r0 + l/16, column l%16; the B operand likewise).  Each of the four
+// waves owns a quarter of every 128-row tile; FP32 products accumulate in
+// the matrix-core registers over one tile (32 rows) and are then added into
+// FP64 registers, FP64 products accumulate in FP64 throughout.  SYM: B is A,
+// read once.  The four waves' 16 x 16 results are summed in LDS in a fixed
+// order and written as this group's partial (entries i < a, j < b).
+// Accumulator layout (per lane l, element v): FP32 16x16x4: row 4 (l/16) + v,
+// column l%16; FP64 16x16x4: row l/16 + 4 v, column l%16.
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+constexpr int MF_TR = 128;       // rows per LDS tile
+constexpr int MF_PITCH = 17;     // LDS row pitch (elements), odd
+
+template <typename R>
+struct MfmaAcc;
+
+template <>
+struct MfmaAcc<float> {
+    mfma_f4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+    double r64[4] = {0, 0, 0, 0}, i64[4] = {0, 0, 0, 0};
+    __device__ __forceinline__ void step(float ar, float ai, float br, float bi, bool conj)
+    {
+        re = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_16x16x4f32(conj ? ai : -ai, bi, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x4f32(conj ? -ai : ai, br, im, 0, 0, 0);
+    }
+    __device__ __forceinline__ void flush()
+    {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            r64[v] += re[v];
+            i64[v] += im[v];
+            re[v] = 0;
+            im[v] = 0;
+        }
+    }
+    __device__ __forceinline__ static int row(int l, int v) { return 4 * (l >> 4) + v; }
+    __device__ __forceinline__ double get_re(int v) const { return r64[v]; }
+    __device__ __forceinline__ double get_im(int v) const { return i64[v]; }
+};
+
+template <>
+struct MfmaAcc<double> {
+    mfma_d4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+    __device__ __forceinline__ void step(double ar, double ai, double br, double bi, bool conj)
+    {
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(conj ? ai : -ai, bi, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(conj ? -ai : ai, br, im, 0, 0, 0);
+    }
+    __device__ __forceinline__ void flush() {}
+    __device__ __forceinline__ static int row(int l, int v) { return (l >> 4) + 4 * v; }
+    __device__ __forceinline__ double get_re(int v) const { return re[v]; }
+    __device__ __forceinline__ double get_im(int v) const { return im[v]; }
+};
+
+template <typename R, bool CONJ, bool SYM>
+__global__ __launch_bounds__(NT_RED) void k_gram_mfma(const cplx<R>* __restrict__ A, int64_t lda, int a,
+                                                     const cplx<R>* __restrict__ B, int64_t ldb, int b,
+                                                     int64_t rows, int64_t rows_per_blk, c64* __restrict__ partial)
+{
+    __shared__ __align__(16) unsigned char smem_raw[(SYM ? 1 : 2) * MF_TR * MF_PITCH * sizeof(cplx<R>) >
+                                                            4 * 256 * sizeof(c64)
+                                                        ? (SYM ? 1 : 2) * MF_TR * MF_PITCH * sizeof(cplx<R>)
+                                                        : 4 * 256 * sizeof(c64)];
+    cplx<R>* As = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* Bs = SYM ? As : As + MF_TR * MF_PITCH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    MfmaAcc<R> acc;
+    const int64_t rb = blockIdx.x * rows_per_blk;
+    const int64_t re = min(rows, rb + rows_per_blk);
+    for (int64_t r0 = rb; r0 < re; r0 += MF_TR) {
+        const int nr = (int)min((int64_t)MF_TR, re - r0);
+        // tile rows beyond nr and columns beyond a (b) are zero
+        for (int e = tid; e < MF_TR * 16; e += NT_RED) {
+            const int r = e % MF_TR, col = e / MF_TR;
+            As[r * MF_PITCH + col] = (r < nr && col < a) ? A[col * lda + r0 + r] : czero<R>();
+            if constexpr (!SYM) Bs[r * MF_PITCH + col] = (r < nr && col < b) ? B[col * ldb + r0 + r] : czero<R>();
+        }
+        __syncthreads();
+        const int col = lane & 15;
+#pragma unroll
+        for (int t = 0; t < MF_TR / 16; ++t) {
+            const int row = wave * (MF_TR / 4) + 4 * t + (lane >> 4);
+            const cplx<R> x = As[row * MF_PITCH + col];
+            const cplx<R> y = Bs[row * MF_PITCH + col];
+            acc.step(x.x, x.y, y.x, y.y, CONJ);
+        }
+        acc.flush();
+        __syncthreads();
+    }
+    // sum the four waves' 16 x 16 results in LDS, wave order fixed
+    c64* red = reinterpret_cast<c64*>(smem_raw);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = MfmaAcc<R>::row(lane, v), j = lane & 15;
+        red[wave * 256 + i * 16 + j] = mk(acc.get_re(v), acc.get_im(v));
+    }
+    __syncthreads();
+    {
+        const int i = tid >> 4, j = tid & 15;
+        double sx = 0, sy = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const c64 v = red[w * 256 + i * 16 + j];
+            sx += v.x;
+            sy += v.y;
+        }
+        if (i < a && j < b) partial[(int64_t)blockIdx.x * a * b + i + (int64_t)j * a] = mk(sx, sy);
+    }
 }
 
 // out[p] = sum_k partial[k np + p]; one 1024-thread group per 64 entries:
@@ -322,6 +442,30 @@ static dim3 grid_cols(int64_t rows, int s)
     return dim3((unsigned)gx, (unsigned)s);
 }
 
+// BDDA_GRAM=valu keeps every Gram on the VALU kernel (the matrix-core one
+// serves blocks up to 16 x 16 otherwise)
+static bool force_valu_gram()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("BDDA_GRAM");
+        return e && std::strcmp(e, "valu") == 0;
+    }();
+    return v;
+}
+
+// reduce the groups' partials and bring the a x b result to the host
+static HostMat finish_gram(BlockWork& w, int nblk, int np)
+{
+    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, 64)), dim3(64 * RED_SLICES), 0, w.stream,
+                       (const c64*)w.partial, nblk, np, w.red);
+    BDDA_HIP(hipGetLastError());
+    HostMat out(np);
+    BDDA_HIP(hipMemcpyAsync(w.host_pinned + w.host_elems, w.red, sizeof(c64) * np, hipMemcpyDeviceToHost, w.stream));
+    BDDA_HIP(hipStreamSynchronize(w.stream));
+    std::memcpy(out.data(), w.host_pinned + w.host_elems, sizeof(c64) * np);
+    return out;
+}
+
 template <typename R>
 HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* B, int64_t ldb, int b,
              int64_t rows, int conjA)
@@ -329,6 +473,27 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
     const int np = a * b;
     const int ntile = ((a + 3) / 4) * ((b + 3) / 4);
     if (a < 1 || b < 1 || np > 4096 || ntile > NT_RED) throw HipError{hipErrorInvalidValue, "gram: block too wide"};
+    if (a <= 16 && b <= 16 && !force_valu_gram()) {
+        // matrix cores; one read of the block when both operands are the same
+        int64_t rpb = std::max<int64_t>(MF_TR, ceil_div(std::max<int64_t>(rows, 1), MAX_PARTIAL_BLOCKS));
+        rpb = ceil_div(rpb, MF_TR) * MF_TR;
+        const int nblk = (int)std::max<int64_t>(1, ceil_div(rows, rpb));
+        w.reserve((size_t)nblk * np, np);
+        const bool sym = (const void*)A == (const void*)B && lda == ldb && a == b;
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(nblk), dim3(NT_RED), 0, w.stream, A, lda, a, B, ldb, b, rows, rpb,
+                               w.partial);
+        };
+        if (conjA) {
+            if (sym) launch(k_gram_mfma<R, true, true>);
+            else launch(k_gram_mfma<R, true, false>);
+        } else {
+            if (sym) launch(k_gram_mfma<R, false, true>);
+            else launch(k_gram_mfma<R, false, false>);
+        }
+        BDDA_HIP(hipGetLastError());
+        return finish_gram(w, nblk, np);
+    }
     const size_t c = sizeof(cplx<R>);
     const int pitch = 4 * ((a + 3) / 4) + 4 * ((b + 3) / 4) + 2;
     int TR = (int)std::min<int64_t>(128, 32768 / (pitch * c));
@@ -345,14 +510,7 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
         hipLaunchKernelGGL((k_gram_partial<R, false>), dim3(nblk), dim3(NT_RED), shm, w.stream, A, lda, a, B, ldb, b,
                            rows, TR, rpb, w.partial);
     BDDA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, 64)), dim3(64 * RED_SLICES), 0, w.stream,
-                       (const c64*)w.partial, nblk, np, w.red);
-    BDDA_HIP(hipGetLastError());
-    HostMat out(np);
-    BDDA_HIP(hipMemcpyAsync(w.host_pinned + w.host_elems, w.red, sizeof(c64) * np, hipMemcpyDeviceToHost, w.stream));
-    BDDA_HIP(hipStreamSynchronize(w.stream));
-    std::memcpy(out.data(), w.host_pinned + w.host_elems, sizeof(c64) * np);
-    return out;
+    return finish_gram(w, nblk, np);
 }
 
 template <typename R>
