@@ -224,9 +224,13 @@ __global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, con
 }
 
 // ---------------------------------------------------------------- z passes
-// A group owns TL consecutive (ix, iy) lines with all F components.  Each
-// component's lines are contiguous in U, so they move between HBM and LDS in
-// coalesced sweeps; the FFT reads/writes them from LDS.
+// A group owns TL consecutive (ix, iy) lines with all F components.  The
+// FFT engine's lanes run over (line, butterfly) with the line fastest, so
+// each of its loads and stores touches TL runs of 64/TL consecutive z
+// positions per wave.  The inverse pass reads U and scatters into the
+// occupied vector straight from the engine's registers.  The forward pass
+// stages the gather through LDS (one contiguous sweep over the cells of the
+// group's lines), which measured faster than gathering per line run.
 
 // forward + scatter: U[((rhs F + c) nxny + l) Pzs + kz] = FFT_z(X[f c2o[cell] + c])
 template <typename R, class PL, int F, int NT, int TL>
@@ -294,42 +298,25 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
     const int64_t l0 = tile * TL;
     const int nl = (int)min((int64_t)TL, nxny - l0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
-    for (int e = threadIdx.x; e < F * TL * P; e += NT) {
-        const int p = e % P;
-        const int rest = e / P;
-        const int tl = rest % TL, c = rest / TL;
-        cplx<R> v = czero<R>();
-        if (tl < nl) v = U[((((int64_t)rhs * F + c) * nxny) + l0 + tl) * Pzs + p];
-        lds[(c * TL + tl) * P + p] = v;
-    }
-    __syncthreads();
+    const cplx<R>* Ur = U + (int64_t)rhs * F * nxny * Pzs;
     PlanFFT<R, PL, NT, TL, F, F> Z(tw, lds);
-    Z.template load<true>([&](int p, int q, int c) { return lds[(c * TL + q) * P + p]; });
-    __syncthreads();
-    Z.template run<true>();
-    // inverse output into LDS [(c TL + line) PZ + p], odd pitch PZ >= nz
-    const int PZ = d.nz | 1;
-    Z.template store<true, true>([&](int p, int q, int c, cplx<R> v) {
-        if (p < d.nz) lds[(c * TL + q) * PZ + p] = v;
+    Z.template load<true>([&](int p, int q, int c) {
+        return q < nl ? Ur[((int64_t)c * nxny + l0 + q) * Pzs + p] : czero<R>();
     });
-    __syncthreads();
+    __syncthreads();  // twiddles
+    Z.template run<true>();
     const cplx<R>* Xr = X + rhs * ldx;
     cplx<R>* Yr = Y + rhs * ldy;
-    // one occupancy lookup per cell; the cells of a line are consecutive
-    for (int e = threadIdx.x; e < nl * d.nz; e += NT) {
-        const int64_t cell = l0 * d.nz + e;
+    Z.template store<true, true>([&](int p, int q, int c, cplx<R> v) {
+        if (p >= d.nz || q >= nl) return;
+        const int64_t cell = (l0 + q) * d.nz + p;
         const int64_t m = c2o ? (int64_t)c2o[cell] : cell;
-        if (m < 0) continue;
-        const int tl = e / d.nz, p = e - tl * d.nz;
-#pragma unroll
-        for (int c = 0; c < F; ++c) {
-            const int64_t i = m * F + c;
-            cplx<R> v = lds[(c * TL + tl) * PZ + p];
-            if (diag) v = diag[i] * Xr[i] + v;
-            if (jpre) v = v * jpre[i];
-            Yr[i] = v;
-        }
-    }
+        if (m < 0) return;
+        const int64_t i = m * F + c;
+        if (diag) v = diag[i] * Xr[i] + v;
+        if (jpre) v = v * jpre[i];
+        Yr[i] = v;
+    });
 }
 
 // ---------------------------------------------------------------- launchers
