@@ -114,7 +114,19 @@ struct PlanTiles {
     // z pass: lines per group (the group's lines x F components x P sit in
     // LDS for the gather / scatter transposition)
     static constexpr int TL_Z = sizeof(R) == 4 ? 4 : 2;
+    // z pass: components per LDS exchange round (fewer rounds with more,
+    // more resident groups with fewer)
+    static constexpr int NCX_Z = 1;
 };
+
+// LDS elements of the z passes beyond the twiddle table
+template <class PL, int F, int TL, int NCX>
+constexpr size_t zpass_lds_elems()
+{
+    const size_t gather = (size_t)TL * F * ((PL::P + 1) / 2 + 1);  // forward gather rows (pitch nz | 1)
+    const size_t xch = (size_t)NCX * TL * PL::P;                   // exchange / store staging
+    return gather > xch ? gather : xch;
+}
 
 // ---------------------------------------------------------------- y pass
 // in[outer][nlin][pitch] -> out[outer][nlout][pitch] over the first Zdim
@@ -147,8 +159,18 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
 
 // ---------------------------------------------------------------- x pass
 // A group owns (rhs, ky, NQ consecutive kz) across all F component planes.
+// Minimum waves per SIMD asked of the compiler.  FP32 plans whose butterfly
+// registers take at most 42 VGPRs (343 = 7^3: one butterfly of 7 values x 3
+// components per thread) fit in 72 VGPRs, so seven waves fit per SIMD (four
+// groups of seven waves per CU instead of three); larger plans would spill.
 template <typename R, class PL, int F, int NT, int NQ>
-__global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, const cplx<R>* __restrict__ Gh,
+constexpr int xfused_min_waves()
+{
+    return (sizeof(R) == 4 && PlanFFT<R, PL, NT, NQ, F, 1>::E * F * 2 <= 42) ? 7 : 1;
+}
+
+template <typename R, class PL, int F, int NT, int NQ>
+__global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_xfused_plan(cplx<R>* __restrict__ V, const cplx<R>* __restrict__ Gh,
                                                     MbtDims d, int Pzs, const cplx<R>* __restrict__ twg, int ntz,
                                                     int nrhs)
 {
@@ -180,14 +202,14 @@ __global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, con
     static_assert(NT % NQ == 0, "a thread keeps one kz for all its butterflies");
     const bool fy = ky > d.Py / 2;
     const int kyr = fy ? d.Py - ky : ky;
-    const int kz = kz0 + (int)(threadIdx.x & (NQ - 1));
+    // (clamped: lanes of a partial kz tile compute on a valid G^ row and never store)
+    const int kz = min(kz0 + (int)(threadIdx.x & (NQ - 1)), d.Pz - 1);
     const bool fz = kz > d.Pz / 2;
     const int kzr = fz ? d.Pz - kz : kz;
     const int off_yz = (kyr * Hz + kzr) * NC;  // < 2^31 for every planned size
     const int stride_x = Hy * Hz * NC;
     const R sy = fy ? R(-1) : R(1), sz = fz ? R(-1) : R(1), syz = sy * sz;
-    X.visit([&](int kx, int q, cplx<R>* h) {
-        if (q >= tqn) return;
+    X.visit([&](int kx, int, cplx<R>* h) {
         const bool fx = kx > P / 2;
         const int kxr = fx ? P - kx : kx;
         const cplx<R>* gp = Gh + (kxr * stride_x + off_yz);
@@ -233,7 +255,7 @@ __global__ __launch_bounds__(NT) void k_xfused_plan(cplx<R>* __restrict__ V, con
 // group's lines), which measured faster than gathering per line run.
 
 // forward + scatter: U[((rhs F + c) nxny + l) Pzs + kz] = FFT_z(X[f c2o[cell] + c])
-template <typename R, class PL, int F, int NT, int TL>
+template <typename R, class PL, int F, int NT, int TL, int NCX>
 __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X, int64_t ldx,
                                                   const int* __restrict__ c2o, cplx<R>* __restrict__ U, MbtDims d,
                                                   int Pzs, const cplx<R>* __restrict__ twg, int64_t ntiles, int nrhs)
@@ -253,9 +275,13 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
     // gather of the occupied values into LDS [(line F + c) PZ + p], one
     // occupancy lookup per cell; rows of odd pitch PZ >= nz keep both the
     // z-contiguous gather and the line-strided FFT loads free of bank conflicts
+    // (nz <= H = (P+1)/2 since P >= 2 nz - 1: the sweep runs over TL rows of
+    // H so that its index split is by a compile-time constant)
     const int PZ = d.nz | 1;
-    for (int e = threadIdx.x; e < TL * d.nz; e += NT) {
-        const int p = e % d.nz, tl = e / d.nz;
+    constexpr int H = (P + 1) / 2;
+    for (int e = threadIdx.x; e < TL * H; e += NT) {
+        const int p = e % H, tl = e / H;
+        if (p >= d.nz) continue;
         int64_t m = -1;
         if (tl < nl) {
             const int64_t cell = (l0 + tl) * d.nz + p;
@@ -265,22 +291,30 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
         for (int c = 0; c < F; ++c) lds[(tl * F + c) * PZ + p] = (m >= 0) ? Xr[m * F + c] : czero<R>();
     }
     __syncthreads();
-    PlanFFT<R, PL, NT, TL, F, F> Z(tw, lds);
+    PlanFFT<R, PL, NT, TL, F, NCX> Z(tw, lds);
     Z.template load<false, true>([&](int p, int q, int c) { return p < d.nz ? lds[(q * F + c) * PZ + p] : czero<R>(); });
     __syncthreads();
     Z.template run<false>();
-    Z.template store<false>([&](int p, int q, int c, cplx<R> v) { lds[(c * TL + q) * P + p] = v; });
-    __syncthreads();
-    for (int e = threadIdx.x; e < F * nl * P; e += NT) {
-        const int p = e % P;
-        const int rest = e / P;
-        const int tl = rest % nl, c = rest / nl;
-        U[((((int64_t)rhs * F + c) * nxny) + l0 + tl) * Pzs + p] = lds[(c * TL + tl) * P + p];
-    }
+    // through LDS, NCX components at a time, so that each wave writes 64
+    // consecutive kz of one row (direct stores from the butterfly registers
+    // write 16-element runs and measured slower in FP32)
+    cplx<R>* Ur = U + ((int64_t)rhs * F * nxny + l0) * Pzs;
+    static_for<0, (F + NCX - 1) / NCX>([&](auto CB) {
+        constexpr int c0 = decltype(CB)::value * NCX;
+        if constexpr (c0 > 0) __syncthreads();  // previous group's copy-out has read LDS
+        Z.template store<false>([&](int p, int q, int c, cplx<R> v) {
+            if (c >= c0 && c < c0 + NCX) lds[((c - c0) * TL + q) * P + p] = v;
+        });
+        __syncthreads();
+        for (int e = threadIdx.x; e < NCX * TL * P; e += NT) {
+            const int p = e % P, tl = (e / P) % TL, c = c0 + e / (P * TL);
+            if (tl < nl && c < F) Ur[((int64_t)c * nxny + tl) * Pzs + p] = lds[e];
+        }
+    });
 }
 
 // inverse + gather: Y[f m + c] = diag .* X + IFFT_z(U)[cell][c]  (then * jpre)
-template <typename R, class PL, int F, int NT, int TL>
+template <typename R, class PL, int F, int NT, int TL, int NCX>
 __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U, const int* __restrict__ c2o,
                                                   const cplx<R>* __restrict__ X, int64_t ldx,
                                                   const cplx<R>* __restrict__ diag, const cplx<R>* __restrict__ jpre,
@@ -299,7 +333,7 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
     const int nl = (int)min((int64_t)TL, nxny - l0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     const cplx<R>* Ur = U + (int64_t)rhs * F * nxny * Pzs;
-    PlanFFT<R, PL, NT, TL, F, F> Z(tw, lds);
+    PlanFFT<R, PL, NT, TL, F, NCX> Z(tw, lds);
     Z.template load<true>([&](int p, int q, int c) {
         return q < nl ? Ur[((int64_t)c * nxny + l0 + q) * Pzs + p] : czero<R>();
     });
@@ -361,9 +395,9 @@ inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx
         using PL = decltype(pl);
         const int64_t nxny = (int64_t)d.nx * d.ny;
         const int64_t ntiles = ceil_div(nxny, T::TL_Z);
-        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::TL_Z * F);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + zpass_lds_elems<PL, F, T::TL_Z, T::NCX_Z>());
         constexpr int NT = plan_threads<PL, T::TL_Z, 2, 256>();
-        hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
+        hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, T::TL_Z, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
                            X, ldx, c2o, U, d, Pzs, tw, ntiles, s);
         BDDA_HIP(hipGetLastError());
     });
@@ -379,9 +413,9 @@ inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X,
         using PL = decltype(pl);
         const int64_t nxny = (int64_t)d.nx * d.ny;
         const int64_t ntiles = ceil_div(nxny, T::TL_Z);
-        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::TL_Z * F);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)T::NCX_Z * T::TL_Z * PL::P);
         constexpr int NT = plan_threads<PL, T::TL_Z, 2, 256>();
-        hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, T::TL_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
+        hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, T::TL_Z, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
                            U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles, s);
         BDDA_HIP(hipGetLastError());
     });
