@@ -44,6 +44,57 @@ __device__ __forceinline__ void static_for(F&& f)
     }
 }
 
+// Odd radix RAD on inputs of which only v[0..NZ) can be nonzero (the first
+// stage of a forward transform of a zero-padded line): the pair sums and
+// differences of DftOdd lose their zero halves at compile time instead of
+// adding +0.0, which the compiler may not drop (x + 0.0 differs from x for
+// x = -0.0).
+template <int RAD, typename R, bool INV, int NZ>
+__device__ __forceinline__ void dft_odd_nz(cplx<R>* v)
+{
+    static constexpr TrigTab<RAD> tab{};
+    constexpr int H = (RAD - 1) / 2;
+    cplx<R> a[H], b[H];
+    static_for<1, H + 1>([&](auto JJ) {
+        constexpr int j = decltype(JJ)::value;
+        if constexpr (j < NZ && RAD - j < NZ) {
+            a[j - 1] = v[j] + v[RAD - j];
+            b[j - 1] = v[j] - v[RAD - j];
+        } else if constexpr (j < NZ) {
+            a[j - 1] = v[j];
+            b[j - 1] = v[j];
+        } else if constexpr (RAD - j < NZ) {
+            a[j - 1] = v[RAD - j];
+            b[j - 1] = -v[RAD - j];
+        }
+    });
+    const cplx<R> x0 = v[0];
+    cplx<R> y0 = x0;
+    static_for<1, H + 1>([&](auto JJ) {
+        constexpr int j = decltype(JJ)::value;
+        if constexpr (j < NZ || RAD - j < NZ) y0 = y0 + a[j - 1];
+    });
+    static_for<1, H + 1>([&](auto KK) {
+        constexpr int k = decltype(KK)::value;
+        cplx<R> A = x0, B = czero<R>();
+        bool first = true;
+        static_for<1, H + 1>([&](auto JJ) {
+            constexpr int j = decltype(JJ)::value;
+            if constexpr (j < NZ || RAD - j < NZ) {
+                const R c = R(tab.c[(j * k) % RAD]);
+                const R sn = R(tab.s[(j * k) % RAD]);
+                A = A + c * a[j - 1];
+                B = first ? sn * b[j - 1] : B + sn * b[j - 1];
+                first = false;
+            }
+        });
+        const cplx<R> iB = mul_pi(B);
+        v[k] = INV ? (A + iB) : (A - iB);
+        v[RAD - k] = INV ? (A - iB) : (A + iB);
+    });
+    v[0] = y0;
+}
+
 // R: real type, PL: Radices<...>, NT threads, NQ lines (power of two), NC
 // components, NCX components per LDS exchange round.  Methods take INV: the
 // inverse transform uses the radices in reverse order, so the forward's last
@@ -83,7 +134,10 @@ struct PlanFFT {
 
     __device__ __forceinline__ PlanFFT(const cplx<R>* tw_, cplx<R>* lds_) : tw(tw_), lds(lds_) {}
 
-    // stage-0 inputs: ld(p, q, c).  HALF: the input is a zero-padded line of
+    // stage-0 inputs: ld(j, RB, q, c) returns element p = j + RB of line q,
+    // component c, where RB (an integral_constant, r P/RAD) is the
+    // compile-time part of p, so that callers can fold it into a uniform
+    // base address.  HALF: the input is a zero-padded line of
     // at most (P+1)/2 values (P >= 2n - 1), so butterfly inputs at
     // p = j + r P/RAD with r P/RAD >= (P+1)/2 are zero whatever j is; they are
     // set at compile time and the arithmetic on them folds away.
@@ -101,14 +155,14 @@ struct PlanFFT {
 #pragma unroll
                     for (int c = 0; c < NC; ++c) {
                         if constexpr (HALF && r * nb >= (P + 1) / 2) v[c][i * RAD + r] = czero<R>();
-                        else v[c][i * RAD + r] = ld(j + r * nb, q, c);
+                        else v[c][i * RAD + r] = ld(j, ic<r * nb>{}, q, c);
                     }
                 });
             }
         });
     }
 
-    // last-stage outputs: st(p, q, c, value).  HALF: only outputs p < (P+1)/2
+    // last-stage outputs: st(j, RB, q, c, value) for p = j + RB.  HALF: only outputs p < (P+1)/2
     // are wanted (the inverse transform keeps n <= (P+1)/2 of them); the
     // others are never stored and their arithmetic is dead code.
     template <bool INV, bool HALF = false, class ST>
@@ -124,7 +178,7 @@ struct PlanFFT {
                     constexpr int r = decltype(RR)::value;
                     if constexpr (!(HALF && r * nb >= (P + 1) / 2)) {
 #pragma unroll
-                        for (int c = 0; c < NC; ++c) st(j + r * nb, q, c, v[c][i * RAD + r]);
+                        for (int c = 0; c < NC; ++c) st(j, ic<r * nb>{}, q, c, v[c][i * RAD + r]);
                     }
                 });
             }
@@ -154,7 +208,8 @@ struct PlanFFT {
         });
     }
 
-    template <int S, bool INV>
+    // NZ: stage-0 butterfly inputs r >= NZ are known zero (HALF loads)
+    template <int S, bool INV, int NZ = 64>
     __device__ __forceinline__ void compute()
     {
         constexpr int RAD = rad<S, INV>(), L = Lof<S, INV>(), nb = P / RAD, total = nb * NQ;
@@ -175,7 +230,10 @@ struct PlanFFT {
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < NC; ++c) Dft<RAD, R, INV>::run(&v[c][i * RAD]);
+                for (int c = 0; c < NC; ++c) {
+                    if constexpr (NZ < RAD && RAD % 2 == 1) dft_odd_nz<RAD, R, INV, NZ>(&v[c][i * RAD]);
+                    else Dft<RAD, R, INV>::run(&v[c][i * RAD]);
+                }
             }
         });
     }
@@ -219,12 +277,15 @@ struct PlanFFT {
         });
     }
 
-    template <bool INV>
+    // HALF: the stage-0 inputs were loaded by load<INV, true>
+    template <bool INV, bool HALF = false>
     __device__ __forceinline__ void run()
     {
+        constexpr int RAD0 = rad<0, INV>(), nb0 = P / RAD0;
+        constexpr int NZ0 = HALF ? ((P + 1) / 2 + nb0 - 1) / nb0 : 64;  // r nb0 < (P+1)/2
         static_for<0, NS>([&](auto SI) {
             constexpr int S = decltype(SI)::value;
-            compute<S, INV>();
+            compute<S, INV, S == 0 ? NZ0 : 64>();
             if constexpr (S + 1 < NS) exchange<S, INV>();
         });
     }

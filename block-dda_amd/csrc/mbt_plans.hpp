@@ -103,6 +103,17 @@ __device__ __forceinline__ void xcd_tile_rhs(int64_t b, int64_t ntiles, int s, i
     }
 }
 
+// base + a 32-bit byte offset: with a uniform base the compiler issues the
+// access as one scalar-base load or store with a 32-bit lane offset, instead
+// of 64-bit address arithmetic per element (the launchers check that the
+// offsets fit)
+template <typename T>
+__device__ __forceinline__ T* at_b(T* base, uint32_t off_bytes)
+{
+    using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T*>(reinterpret_cast<C*>(base) + off_bytes);
+}
+
 // tile widths of the planned passes (row segments of 32 to 128 bytes)
 template <typename R>
 struct PlanTiles {
@@ -147,13 +158,17 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
     const cplx<R>* src = in + (outer * nlin) * pitch + z0;
     cplx<R>* dst = out + (outer * nlout) * pitch + z0;
     PlanFFT<R, PL, NT, NQ, 1, 1> F(tw, lds);
-    F.template load<INV, !INV>([&](int p, int q, int) {
-        return (p < nlin && q < tz) ? src[(int64_t)p * pitch + q] : czero<R>();
+    const uint32_t pitch32 = (uint32_t)pitch;
+    auto lane_b = [&](int j, int q) { return ((uint32_t)j * pitch32 + (uint32_t)q) * (uint32_t)sizeof(cplx<R>); };
+    F.template load<INV, !INV>([&](int j, auto RB, int q, int) {
+        const int p = j + decltype(RB)::value;
+        return (p < nlin && q < tz) ? *at_b(src + (int64_t)decltype(RB)::value * pitch, lane_b(j, q)) : czero<R>();
     });
     __syncthreads();
-    F.template run<INV>();
-    F.template store<INV, INV>([&](int p, int q, int, cplx<R> v) {
-        if (p < nlout && q < tz) dst[(int64_t)p * pitch + q] = v;
+    F.template run<INV, !INV>();
+    F.template store<INV, INV>([&](int j, auto RB, int q, int, cplx<R> v) {
+        const int p = j + decltype(RB)::value;
+        if (p < nlout && q < tz) *at_b(dst + (int64_t)decltype(RB)::value * pitch, lane_b(j, q)) = v;
     });
 }
 
@@ -190,11 +205,21 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     cplx<R>* base = V + (int64_t)rhs * F * cstride + (int64_t)ky * Pzs + kz0;
     PlanFFT<R, PL, NT, NQ, F, 1> X(tw, lds);
-    X.template load<false, true>([&](int p, int q, int c) {
-        return (p < d.nx && q < tqn) ? base[c * cstride + p * plane + q] : czero<R>();
+    // V addresses split into a uniform row base (component, compile-time row
+    // offset RB) and a 32-bit per-lane byte offset (j plane + q, j < P/r0;
+    // the launcher checks that it fits), so each access is a scalar-base
+    // load or store
+    const uint32_t plane32 = (uint32_t)plane;
+    auto vaddr = [&](int j, auto RB, int q, int c) {
+        char* row = reinterpret_cast<char*>(base + (c * cstride + (int64_t)decltype(RB)::value * plane));
+        return reinterpret_cast<cplx<R>*>(row + ((uint32_t)j * plane32 + (uint32_t)q) * (uint32_t)sizeof(cplx<R>));
+    };
+    X.template load<false, true>([&](int j, auto RB, int q, int c) {
+        const int p = j + decltype(RB)::value;
+        return (p < d.nx && q < tqn) ? *vaddr(j, RB, q, c) : czero<R>();
     });
     __syncthreads();
-    X.template run<false>();
+    X.template run<false, true>();
     // G^ is stored for one octant; the (kx, ky, kz) image of a reduced element
     // carries the signs of the axis reflections: G'_ab = s_a s_b G_ab for the
     // symmetric part, and det(s) s_a s_b A_ab for the antisymmetric part of f = 6
@@ -212,7 +237,8 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
     X.visit([&](int kx, int, cplx<R>* h) {
         const bool fx = kx > P / 2;
         const int kxr = fx ? P - kx : kx;
-        const cplx<R>* gp = Gh + (kxr * stride_x + off_yz);
+        const cplx<R>* gp = reinterpret_cast<const cplx<R>*>(
+            reinterpret_cast<const char*>(Gh) + (uint32_t)(kxr * stride_x + off_yz) * (uint32_t)sizeof(cplx<R>));
         const R sx = fx ? R(-1) : R(1), sxy = sx * sy, sxz = sx * sz;
         cplx<R> gv[NC];
 #pragma unroll
@@ -240,8 +266,9 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
         h[2] = oz;
     });
     X.template run<true>();
-    X.template store<true, true>([&](int p, int q, int c, cplx<R> v) {
-        if (p < d.nx && q < tqn) base[c * cstride + p * plane + q] = v;
+    X.template store<true, true>([&](int j, auto RB, int q, int c, cplx<R> v) {
+        const int p = j + decltype(RB)::value;
+        if (p < d.nx && q < tqn) *vaddr(j, RB, q, c) = v;
     });
 }
 
@@ -279,22 +306,28 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
     // H so that its index split is by a compile-time constant)
     const int PZ = d.nz | 1;
     constexpr int H = (P + 1) / 2;
+    const int cell0 = (int)l0 * d.nz;
     for (int e = threadIdx.x; e < TL * H; e += NT) {
         const int p = e % H, tl = e / H;
         if (p >= d.nz) continue;
-        int64_t m = -1;
+        int m = -1;
         if (tl < nl) {
-            const int64_t cell = (l0 + tl) * d.nz + p;
-            m = c2o ? (int64_t)c2o[cell] : cell;
+            const int cell = cell0 + tl * d.nz + p;
+            m = c2o ? c2o[cell] : cell;
         }
+        const uint32_t ob = (uint32_t)m * (uint32_t)(F * sizeof(cplx<R>));
 #pragma unroll
-        for (int c = 0; c < F; ++c) lds[(tl * F + c) * PZ + p] = (m >= 0) ? Xr[m * F + c] : czero<R>();
+        for (int c = 0; c < F; ++c)
+            lds[(tl * F + c) * PZ + p] = (m >= 0) ? *at_b(Xr + c, ob) : czero<R>();
     }
     __syncthreads();
     PlanFFT<R, PL, NT, TL, F, NCX> Z(tw, lds);
-    Z.template load<false, true>([&](int p, int q, int c) { return p < d.nz ? lds[(q * F + c) * PZ + p] : czero<R>(); });
+    Z.template load<false, true>([&](int j, auto RB, int q, int c) {
+        const int p = j + decltype(RB)::value;
+        return p < d.nz ? lds[(q * F + c) * PZ + p] : czero<R>();
+    });
     __syncthreads();
-    Z.template run<false>();
+    Z.template run<false, true>();
     // through LDS, NCX components at a time, so that each wave writes 64
     // consecutive kz of one row (direct stores from the butterfly registers
     // write 16-element runs and measured slower in FP32)
@@ -302,13 +335,15 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
     static_for<0, (F + NCX - 1) / NCX>([&](auto CB) {
         constexpr int c0 = decltype(CB)::value * NCX;
         if constexpr (c0 > 0) __syncthreads();  // previous group's copy-out has read LDS
-        Z.template store<false>([&](int p, int q, int c, cplx<R> v) {
-            if (c >= c0 && c < c0 + NCX) lds[((c - c0) * TL + q) * P + p] = v;
+        Z.template store<false>([&](int j, auto RB, int q, int c, cplx<R> v) {
+            if (c >= c0 && c < c0 + NCX) lds[((c - c0) * TL + q) * P + j + decltype(RB)::value] = v;
         });
         __syncthreads();
         for (int e = threadIdx.x; e < NCX * TL * P; e += NT) {
-            const int p = e % P, tl = (e / P) % TL, c = c0 + e / (P * TL);
-            if (tl < nl && c < F) Ur[((int64_t)c * nxny + tl) * Pzs + p] = lds[e];
+            const int p = e % P, tl = (e / P) % TL, cc = e / (P * TL);
+            if (tl < nl && c0 + cc < F)
+                *at_b(Ur + (int64_t)(c0 + cc) * nxny * Pzs, ((uint32_t)tl * (uint32_t)Pzs + (uint32_t)p) *
+                                                                (uint32_t)sizeof(cplx<R>)) = lds[e];
         }
     });
 }
@@ -332,24 +367,28 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
     const int64_t l0 = tile * TL;
     const int nl = (int)min((int64_t)TL, nxny - l0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
-    const cplx<R>* Ur = U + (int64_t)rhs * F * nxny * Pzs;
+    const cplx<R>* Ur = U + ((int64_t)rhs * F * nxny + l0) * Pzs;
     PlanFFT<R, PL, NT, TL, F, NCX> Z(tw, lds);
-    Z.template load<true>([&](int p, int q, int c) {
-        return q < nl ? Ur[((int64_t)c * nxny + l0 + q) * Pzs + p] : czero<R>();
+    Z.template load<true>([&](int j, auto RB, int q, int c) {
+        const cplx<R>* row = Ur + ((int64_t)c * nxny * Pzs + decltype(RB)::value);
+        return q < nl ? *at_b(row, ((uint32_t)q * (uint32_t)Pzs + (uint32_t)j) * (uint32_t)sizeof(cplx<R>))
+                      : czero<R>();
     });
     __syncthreads();  // twiddles
     Z.template run<true>();
     const cplx<R>* Xr = X + rhs * ldx;
     cplx<R>* Yr = Y + rhs * ldy;
-    Z.template store<true, true>([&](int p, int q, int c, cplx<R> v) {
+    const int cell0 = (int)l0 * d.nz;
+    Z.template store<true, true>([&](int j, auto RB, int q, int c, cplx<R> v) {
+        const int p = j + decltype(RB)::value;
         if (p >= d.nz || q >= nl) return;
-        const int64_t cell = (l0 + q) * d.nz + p;
-        const int64_t m = c2o ? (int64_t)c2o[cell] : cell;
+        const int cell = cell0 + q * d.nz + p;
+        const int m = c2o ? c2o[cell] : cell;
         if (m < 0) return;
-        const int64_t i = m * F + c;
-        if (diag) v = diag[i] * Xr[i] + v;
-        if (jpre) v = v * jpre[i];
-        Yr[i] = v;
+        const uint32_t ob = (uint32_t)m * (uint32_t)(F * sizeof(cplx<R>));
+        if (diag) v = *at_b(diag + c, ob) * *at_b(Xr + c, ob) + v;
+        if (jpre) v = v * *at_b(jpre + c, ob);
+        *at_b(Yr + c, ob) = v;
     });
 }
 
@@ -359,6 +398,7 @@ inline bool launch_lines_plan(const cplx<R>* in, cplx<R>* out, int nlin, int nlo
                               int64_t outer, int P, const cplx<R>* tw, hipStream_t st)
 {
     using T = PlanTiles<R>;
+    if ((uint64_t)P * pitch * sizeof(cplx<R>) >= (1ull << 32)) return false;  // 32-bit lane offsets
     return with_plan(P, [&](auto pl) {
         using PL = decltype(pl);
         const int64_t ntiles = ceil_div(Zdim, T::NQ_Y);
@@ -375,8 +415,17 @@ inline bool launch_xfused_plan(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, 
                                hipStream_t st)
 {
     using T = PlanTiles<R>;
-    return with_plan(d.Px, [&](auto pl) {
+    bool fits = true;
+    const bool planned = with_plan(d.Px, [&](auto pl) {
         using PL = decltype(pl);
+        // per-lane byte offsets into V (within a row block) and G^ are 32-bit
+        // (the all-in-LDS kernels take grids beyond that)
+        const uint64_t gh_bytes = (uint64_t)(d.Px / 2 + 1) * (d.Py / 2 + 1) * (d.Pz / 2 + 1) * (F == 3 ? 6 : 9) *
+                                  sizeof(cplx<R>);
+        if ((uint64_t)(PL::P / PL::r[0]) * d.Py * Pzs * sizeof(cplx<R>) >= (1ull << 32) || gh_bytes >= (1ull << 32)) {
+            fits = false;
+            return;
+        }
         const int ntz = (int)ceil_div(d.Pz, T::NQ_X);
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_X);
         constexpr int NT = plan_threads<PL, T::NQ_X, 2, 512>();
@@ -384,6 +433,7 @@ inline bool launch_xfused_plan(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, 
                            dim3(NT), shm, st, V, gh, d, Pzs, tw, ntz, s);
         BDDA_HIP(hipGetLastError());
     });
+    return planned && fits;
 }
 
 template <typename R, int F>
@@ -391,6 +441,9 @@ inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx
                              const cplx<R>* tw, int s, hipStream_t st)
 {
     using T = PlanTiles<R>;
+    // 32-bit cell indices and lane offsets into the occupied rows
+    const uint64_t cells = (uint64_t)d.nx * d.ny * d.nz;
+    if (cells >= (1ull << 31) || cells * F * sizeof(cplx<R>) >= (1ull << 32)) return false;
     return with_plan(d.Pz, [&](auto pl) {
         using PL = decltype(pl);
         const int64_t nxny = (int64_t)d.nx * d.ny;
@@ -409,6 +462,9 @@ inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X,
                              const cplx<R>* tw, int s, hipStream_t st)
 {
     using T = PlanTiles<R>;
+    // 32-bit cell indices and lane offsets into the occupied rows
+    const uint64_t cells = (uint64_t)d.nx * d.ny * d.nz;
+    if (cells >= (1ull << 31) || cells * F * sizeof(cplx<R>) >= (1ull << 32)) return false;
     return with_plan(d.Pz, [&](auto pl) {
         using PL = decltype(pl);
         const int64_t nxny = (int64_t)d.nx * d.ny;
