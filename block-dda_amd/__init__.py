@@ -96,6 +96,8 @@ SIGNATURES = {
     "bdda_mbt_algorithmic_bytes": (ctypes.c_double, [_vp, ctypes.c_int, ctypes.c_int]),
     "bdda_cross_sections": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "bdda_cross_sections_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
+    "bdda_residual_columns": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int, _dp]),
+    "bdda_residual_columns_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _dp]),
     "bdda_scattering_matrix": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp,
                                               _vp]),
     "bdda_scattering_matrix_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _vp]),
@@ -264,6 +266,15 @@ class MbtOperator:
             _check(rc)
         return X, info.as_dict(), hist[:hist_len]
 
+
+    def residual_columns(self, B, X):
+        """Per-column FP64 true residuals ||B_j - A X_j|| / ||B_j|| (one FP64 block matvec)."""
+        B = _cz(B).reshape(B.shape[0], -1, order="F")
+        X = _cz(X).reshape(X.shape[0], -1, order="F")
+        res = np.zeros(B.shape[1])
+        _check(lib().bdda_residual_columns(self.h, _ptr(B), B.shape[0], _ptr(X), X.shape[0], B.shape[1],
+                                           res.ctypes.data_as(_dp)))
+        return res
 
     def set_workspace_limit(self, nbytes):
         _check(lib().bdda_mbt_set_workspace_limit(self.h, int(nbytes)))

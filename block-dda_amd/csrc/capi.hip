@@ -428,6 +428,36 @@ int bdda_solve_dev(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64
     })
 }
 
+static int rescol_dev(bdda_mbt* h, const c64* dB, const c64* dX, int L, double* res)
+{
+    if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
+    if (L < 1 || L > 64 || !res) return fail(BDDA_EINVAL, "need 1 <= L <= 64 and an output array");
+    residual_columns(h->prob, dB, dX, L, res);
+    return BDDA_OK;
+}
+
+int bdda_residual_columns(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdda_c64* X, int64_t ldx, int L,
+                          double* res)
+{
+    if (!h || !B || !X) return fail(BDDA_EINVAL, "null pointer");
+    BDDA_TRY({
+        if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
+        const int64_t rows = h->prob.rows();
+        if (L < 1 || ldb < rows || ldx < rows) return fail(BDDA_EINVAL, "bad block shape");
+        DevMem<c64> dB((size_t)rows * L);
+        DevMem<c64> dX((size_t)rows * L);
+        to_dev<double>(B, ldb, rows, L, dB.p, h->stream);
+        to_dev<double>(X, ldx, rows, L, dX.p, h->stream);
+        return rescol_dev(h, dB.p, dX.p, L, res);
+    })
+}
+
+int bdda_residual_columns_dev(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX, int L, double* res)
+{
+    if (!h || !dB || !dX) return fail(BDDA_EINVAL, "null pointer");
+    BDDA_TRY({ return rescol_dev(h, reinterpret_cast<const c64*>(dB), reinterpret_cast<const c64*>(dX), L, res); })
+}
+
 int bdda_cross_sections(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdda_c64* X, int64_t ldx, int L,
                         int alpha_mode, double* Cext, double* Cabs, double* Csca)
 {

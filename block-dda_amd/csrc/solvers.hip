@@ -391,6 +391,27 @@ double true_residual(Problem& P, const c64* B64, const c64* X64, int s)
     return rn / bn;
 }
 
+void residual_columns(Problem& P, const c64* B64, const c64* X64, int s, double* out)
+{
+    const int64_t rows = P.rows(), N = rows * s;
+    BlockWork& w = P.work;
+    DevBuf<c64> AX(N);
+    problem_matvec<double>(P, X64, AX.p, s, 0);
+    Combo<double> cb;
+    cb.nv = 2;
+    cb.V[0] = B64; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
+    cb.V[1] = AX.p; cb.ldv[1] = rows; cb.a[1] = mk(-1.0, 0.0);
+    combo<double>(w, cb, AX.p, rows, rows, s);
+    // column norms from the diagonals of R^H R and B^H B
+    const HostMat gr = gram<double>(w, AX.p, rows, s, AX.p, rows, s, rows, 1);
+    const HostMat gb = gram<double>(w, B64, rows, s, B64, rows, s, rows, 1);
+    for (int j = 0; j < s; ++j) {
+        const double rn = std::sqrt(std::max(gr[(size_t)j * s + j].x, 0.0));
+        const double bn = std::sqrt(std::max(gb[(size_t)j * s + j].x, 0.0));
+        out[j] = bn > 0 ? rn / bn : rn;
+    }
+}
+
 int solve_fp(Problem& P, int method, int jacobi, int mode, const c64* B64, c64* X64, int s, double tol,
              int itermax, SolveInfo& info, std::vector<double>* hist, double inner_tol)
 {

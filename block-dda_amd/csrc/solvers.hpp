@@ -52,4 +52,7 @@ int solve_fp(Problem& P, int method, int jacobi, int mode, const c64* B64, c64* 
 // FP64 true relative residual ||B - A X|| / ||B|| (uses one FP64 matvec)
 double true_residual(Problem& P, const c64* B64, const c64* X64, int s);
 
+// per column: ||B_j - A X_j|| / ||B_j|| (||B_j - A X_j|| where B_j = 0), one FP64 matvec
+void residual_columns(Problem& P, const c64* B64, const c64* X64, int s, double* out);
+
 }  // namespace bdda

@@ -114,6 +114,14 @@ int bdda_mbt_apply_occupied(bdda_mbt* h, int prec, int jacobi, const bdda_c64* X
 int bdda_solve(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64* B, int64_t ldb, int L, double tol,
                int itermax, bdda_c64* X, int64_t ldx, bdda_solve_info* info, double* hist, int hist_len);
 
+/* Per-column true residuals res[j] = ||B_j - A X_j|| / ||B_j|| (||B_j - A X_j||
+ * when B_j = 0), in FP64 with one FP64 block matvec and A without Jacobi
+ * scaling.  No reference counterpart: the reference prints only the block
+ * recurrence residual ||xi||_F / ||B||_F (bl_cocg_rq_mvp_fft.cpp:58-60), so
+ * a block can pass with one column above tol; this reports the worst one. */
+int bdda_residual_columns(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdda_c64* X, int64_t ldx, int L,
+                          double* res);
+
 /* qr_reduced (bl_krylov_mvp_fft/qr_reduced.cpp:9-31) on the GPU: A is m x n
  * column-major (ld = m); Q (m x n) and R (n x n, R(i,k) first-pass
  * coefficients, R(k,k) the norm after re-orthogonalisation). */
@@ -183,7 +191,9 @@ int bdda_solve_dev(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64
 /* device block matvec in the given precision; dX/dY hold complex<float>
  * (prec FP32) or complex<double> (prec FP64), ld = f*nocc */
 int bdda_mbt_apply_occupied_dev(bdda_mbt* h, int prec, int jacobi, const void* dX, void* dY, int ncols);
-/* bdda_cross_sections / bdda_scattering_matrix on device blocks (ld = f*nocc) */
+/* bdda_residual_columns, bdda_cross_sections and bdda_scattering_matrix on
+ * device blocks (ld = f*nocc) */
+int bdda_residual_columns_dev(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX, int L, double* res);
 int bdda_cross_sections_dev(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX, int L, int alpha_mode, double* Cext,
                             double* Cabs, double* Csca);
 int bdda_scattering_matrix_dev(bdda_mbt* h, const bdda_c64* dX, int npairs, const double* u, const double* e1,

@@ -81,6 +81,31 @@ def test_mixed_precision_reaches_fp64_accuracy(bdda, orc, method):
     assert np.linalg.norm(X - Xd) / np.linalg.norm(Xd) < 1e-7
 
 
+def test_residual_columns_against_dense(bdda, orc):
+    """Per-column true residuals: an FP32 solve (columns converge unevenly)
+    and an arbitrary X with a zero column of B (absolute residual there)."""
+    n, addr, diag, B, lf, k = problem(bdda, 3, seed=5, L=5)
+    op = bdda.MBT_fft_init(n, 3, lf, k)
+    op.set_target(addr, diag)
+    A = orc.dense_system(n, 3, lf, k, addr, diag)
+    X, info, _ = op.solve(B, 1e-4, 400, mode=bdda.MODE_FP32)
+    res = op.residual_columns(B, X)
+    ref = np.linalg.norm(A @ X - B, axis=0) / np.linalg.norm(B, axis=0)
+    assert np.allclose(res, ref, rtol=1e-8, atol=0)
+    # the Frobenius figure of the block follows from the columns
+    fro = np.sqrt(np.sum((res * np.linalg.norm(B, axis=0)) ** 2)) / np.linalg.norm(B)
+    assert abs(fro - np.linalg.norm(A @ X - B) / np.linalg.norm(B)) <= 1e-10 * fro
+    rng = np.random.default_rng(11)
+    X2 = rng.standard_normal(B.shape) + 1j * rng.standard_normal(B.shape)
+    B2 = B.copy()
+    B2[:, 1] = 0
+    res2 = op.residual_columns(B2, X2)
+    R2 = np.linalg.norm(A @ X2 - B2, axis=0)
+    nb = np.linalg.norm(B2, axis=0)
+    ref2 = np.where(nb > 0, R2 / np.where(nb > 0, nb, 1), R2)
+    assert np.allclose(res2, ref2, rtol=1e-10, atol=0)
+
+
 def test_fp32_mode_is_fp32_accurate(bdda, orc):
     n, addr, diag, B, lf, k = problem(bdda, 3, seed=6)
     op = bdda.MBT_fft_init(n, 3, lf, k)
