@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE counter runs scaled by a 2 GiB device copy
 # (profiles/mbt_matvec_agg16384_hbm_traffic.md); counters cannot be read
 # from inside this process, so the figure is carried here per workload.
-MEASURED_X_TRAFFIC = {"agg16384": 24.14e9}
+MEASURED_X_TRAFFIC = {"agg16384": 17.29e9}
 
 
 def load_pkg():

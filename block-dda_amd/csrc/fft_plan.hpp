@@ -1,7 +1,7 @@
 // fft_plan.hpp -- compile-time-planned register-resident Stockham FFT.
 //
 // A work-group of NT threads transforms NQ lines (length P) of NC
-// components; NQ (a power of two) indexes the contiguous axis in HBM, so a
+// components; NQ indexes the contiguous axis in HBM, so a
 // wave reading butterfly inputs x[j + r P/RAD][q][c] touches consecutive
 // addresses.  Data stays in registers: stage-0 inputs come straight from
 // the caller's loader, twiddles and the radix-RAD DFT run in registers, and
@@ -125,8 +125,11 @@ struct PlanFFT {
         return m;
     }
     static constexpr int E = emax();
-    static constexpr int QS = (NQ == 1) ? 0 : (NQ == 2) ? 1 : (NQ == 4) ? 2 : (NQ == 8) ? 3 : (NQ == 16) ? 4 : (NQ == 32) ? 5 : -1;
-    static_assert(QS >= 0, "NQ must be a power of two <= 32");
+    // butterfly b -> (line q, butterfly j); NQ need not be a power of two (a
+    // line count that fills whole waves may be odd), and the split is by a
+    // compile-time constant
+    __device__ __forceinline__ static int qof(int b) { return (int)((unsigned)b % (unsigned)NQ); }
+    __device__ __forceinline__ static int jof(int b) { return (int)((unsigned)b / (unsigned)NQ); }
 
     cplx<R> v[NC][E];
     const cplx<R>* tw;  // LDS twiddles, tw[i] = exp(-2 pi i i / P)
@@ -149,7 +152,7 @@ struct PlanFFT {
             constexpr int i = decltype(I)::value;
             const int b = threadIdx.x + i * NT;
             if (total % NT == 0 || b < total) {
-                const int q = b & (NQ - 1), j = b >> QS;
+                const int q = qof(b), j = jof(b);
                 static_for<0, RAD>([&](auto RR) {
                     constexpr int r = decltype(RR)::value;
 #pragma unroll
@@ -173,7 +176,7 @@ struct PlanFFT {
             constexpr int i = decltype(I)::value;
             const int b = threadIdx.x + i * NT;
             if (total % NT == 0 || b < total) {
-                const int q = b & (NQ - 1), j = b >> QS;
+                const int q = qof(b), j = jof(b);
                 static_for<0, RAD>([&](auto RR) {
                     constexpr int r = decltype(RR)::value;
                     if constexpr (!(HALF && r * nb >= (P + 1) / 2)) {
@@ -194,7 +197,7 @@ struct PlanFFT {
             constexpr int i = decltype(I)::value;
             const int b = threadIdx.x + i * NT;
             if (total % NT == 0 || b < total) {
-                const int q = b & (NQ - 1), j = b >> QS;
+                const int q = qof(b), j = jof(b);
 #pragma unroll
                 for (int r = 0; r < RAD; ++r) {
                     cplx<R> x[NC];
@@ -219,7 +222,7 @@ struct PlanFFT {
             const int b = threadIdx.x + i * NT;
             if (total % NT == 0 || b < total) {
                 if constexpr (L > 1) {
-                    const int j = b >> QS;
+                    const int j = jof(b);
                     const int kt = (j % L) * tstep;
 #pragma unroll
                     for (int r = 1; r < RAD; ++r) {
@@ -251,14 +254,14 @@ struct PlanFFT {
                 constexpr int i = decltype(I)::value;
                 const int b = threadIdx.x + i * NT;
                 if (to % NT == 0 || b < to) {
-                    const int q = b & (NQ - 1), j = b >> QS;
+                    const int q = qof(b), j = jof(b);
                     const int k = j % L;
-                    const int base = (((j - k) * RO + k) << QS) + q;
+                    const int base = ((j - k) * RO + k) * NQ + q;
 #pragma unroll
                     for (int r = 0; r < RO; ++r)
 #pragma unroll
                         for (int cc = 0; cc < NCX; ++cc)
-                            if (c0 + cc < NC) lds[cc * PQ + base + ((r * L) << QS)] = v[c0 + cc][i * RO + r];
+                            if (c0 + cc < NC) lds[cc * PQ + base + r * L * NQ] = v[c0 + cc][i * RO + r];
                 }
             });
             __syncthreads();
@@ -270,7 +273,7 @@ struct PlanFFT {
                     for (int r = 0; r < RI; ++r)
 #pragma unroll
                         for (int cc = 0; cc < NCX; ++cc)
-                            if (c0 + cc < NC) v[c0 + cc][i * RI + r] = lds[cc * PQ + b + ((r * nbi) << QS)];
+                            if (c0 + cc < NC) v[c0 + cc][i * RI + r] = lds[cc * PQ + b + r * nbi * NQ];
                 }
             });
             __syncthreads();

@@ -82,25 +82,63 @@ constexpr int plan_threads()
     return best == 0 ? 1024 : best;
 }
 
+// Fraction of issued butterfly lanes that do work, at plan_threads' choice.
+template <class PL, int NQ, int MB, int TARGET>
+constexpr double plan_fill()
+{
+    const int nt = plan_threads<PL, NQ, MB, TARGET>();
+    long used = 0, slots = 0;
+    for (int s = 0; s < PL::N; ++s) {
+        const long b = (long)(PL::P / PL::r[s]) * NQ;
+        used += b;
+        slots += ((b + nt - 1) / nt) * nt;
+    }
+    return (double)used / (double)slots;
+}
+
+// Lines per group: the base count, or one more when that fills the waves
+// better by over 5 % and every stage still takes one round (so a thread keeps
+// one line throughout).  343 = 7^3 has 49 butterflies per line and stage:
+// 4 lines fill 3.06 of 4 waves (76.6 % of the lanes work), 5 lines 3.83
+// (95.7 %).  Used by the z inverse pass, whose lines are whole rows.
+template <class PL, int BASE, int MB, int TARGET>
+constexpr int plan_lines()
+{
+    constexpr int alt = BASE + 1;
+    constexpr int nt = plan_threads<PL, alt, MB, TARGET>();
+    bool one_round = true;
+    for (int s = 0; s < PL::N; ++s) one_round = one_round && (long)(PL::P / PL::r[s]) * alt <= nt;
+    return (one_round && plan_fill<PL, alt, MB, TARGET>() > plan_fill<PL, BASE, MB, TARGET>() + 0.05) ? alt : BASE;
+}
+
 // Work-group -> (tile, right-hand side).  Work-groups are dispatched round-
 // robin over the 8 XCDs (group b runs on XCD b mod 8), each with its own L2.
-// Tiles are dealt to XCDs in the same round-robin, and on each XCD the s
-// right-hand sides of a tile run back to back, so the per-tile data they
-// share (G^, the occupancy map, DIAG) comes from HBM once per tile and from
-// that XCD's L2 for the other s - 1.  A bijection on [0, ntiles * s).
+// Runs of PAIR consecutive tiles are dealt to XCDs in the same round-robin,
+// and on each XCD the s right-hand sides of a run go back to back, the PAIR
+// tiles of one right-hand side next to each other.  So the per-tile data the
+// right-hand sides share (G^, the occupancy map, DIAG) comes from HBM once
+// per tile, and with PAIR = 2 two half-line row segments of adjacent tiles
+// (64-byte runs of a 128-byte L2 line) are read through the same L2.  A
+// bijection on [0, ntiles * s) for ntiles a multiple of PAIR.
+template <int PAIR = 1>
 __device__ __forceinline__ void xcd_tile_rhs(int64_t b, int64_t ntiles, int s, int64_t& tile, int& rhs)
 {
     constexpr int NXCD = 8;
-    const int64_t full = (ntiles / NXCD) * NXCD * s;
+    const int64_t nrun = ntiles / PAIR;
+    const int64_t per = (int64_t)PAIR * s;  // groups per run
+    const int64_t full = (nrun / NXCD) * NXCD * per;
+    int64_t run, w;
     if (b < full) {
         const int64_t local = b / NXCD;
-        tile = (local / s) * NXCD + b % NXCD;
-        rhs = (int)(local % s);
+        run = (local / per) * NXCD + b % NXCD;
+        w = local % per;
     } else {
         const int64_t r = b - full;
-        tile = (ntiles / NXCD) * NXCD + r / s;
-        rhs = (int)(r % s);
+        run = (nrun / NXCD) * NXCD + r / per;
+        w = r % per;
     }
+    rhs = (int)(w / PAIR);
+    tile = run * PAIR + w % PAIR;
 }
 
 // base + a 32-bit byte offset: with a uniform base the compiler issues the
@@ -198,9 +236,12 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
     const int64_t cstride = (int64_t)d.nx * plane;
     int64_t tile;
     int rhs;
-    xcd_tile_rhs(blockIdx.x, (int64_t)d.Py * ntz, nrhs, tile, rhs);
-    const int ky = (int)(tile / ntz);
-    const int kz0 = (int)(tile % ntz) * NQ;
+    // kz tiles in pairs (ntz rounded up to even; the padding groups exit)
+    const int ntzp = (ntz + 1) & ~1;
+    xcd_tile_rhs<2>(blockIdx.x, (int64_t)d.Py * ntzp, nrhs, tile, rhs);
+    const int ky = (int)(tile / ntzp);
+    const int kz0 = (int)(tile % ntzp) * NQ;
+    if (kz0 >= d.Pz) return;  // whole group, before any barrier
     const int tqn = min(NQ, d.Pz - kz0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     cplx<R>* base = V + (int64_t)rhs * F * cstride + (int64_t)ky * Pzs + kz0;
@@ -224,11 +265,12 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
     // carries the signs of the axis reflections: G'_ab = s_a s_b G_ab for the
     // symmetric part, and det(s) s_a s_b A_ab for the antisymmetric part of f = 6
     const int Hy = d.Py / 2 + 1, Hz = d.Pz / 2 + 1;
-    static_assert(NT % NQ == 0, "a thread keeps one kz for all its butterflies");
+    static_assert(NT % NQ == 0 || PlanFFT<R, PL, NT, NQ, F, 1>::template mb<PL::N - 1, false>() == 1,
+                  "a thread keeps one kz for all its butterflies");
     const bool fy = ky > d.Py / 2;
     const int kyr = fy ? d.Py - ky : ky;
     // (clamped: lanes of a partial kz tile compute on a valid G^ row and never store)
-    const int kz = min(kz0 + (int)(threadIdx.x & (NQ - 1)), d.Pz - 1);
+    const int kz = min(kz0 + (int)(threadIdx.x % NQ), d.Pz - 1);
     const bool fz = kz > d.Pz / 2;
     const int kzr = fz ? d.Pz - kz : kz;
     const int off_yz = (kyr * Hz + kzr) * NC;  // < 2^31 for every planned size
@@ -426,10 +468,13 @@ inline bool launch_xfused_plan(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, 
             fits = false;
             return;
         }
-        const int ntz = (int)ceil_div(d.Pz, T::NQ_X);
-        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_X);
-        constexpr int NT = plan_threads<PL, T::NQ_X, 2, 512>();
-        hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, T::NQ_X>), dim3((unsigned)((int64_t)s * d.Py * ntz)),
+        // (odd line counts that fill the waves, 9 or 5 at 343, measured 19 % and
+        // 14 % slower here: their row segments straddle 64-byte boundaries)
+        constexpr int NQ = T::NQ_X;
+        const int ntz = (int)ceil_div(d.Pz, NQ);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * NQ);
+        constexpr int NT = plan_threads<PL, NQ, 2, 512>();
+        hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, NQ>), dim3((unsigned)((int64_t)s * d.Py * ((ntz + 1) & ~1))),
                            dim3(NT), shm, st, V, gh, d, Pzs, tw, ntz, s);
         BDDA_HIP(hipGetLastError());
     });
@@ -446,11 +491,12 @@ inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx
     if (cells >= (1ull << 31) || cells * F * sizeof(cplx<R>) >= (1ull << 32)) return false;
     return with_plan(d.Pz, [&](auto pl) {
         using PL = decltype(pl);
+        constexpr int TL = T::TL_Z;  // more lines would cut the resident groups (LDS gather rows)
         const int64_t nxny = (int64_t)d.nx * d.ny;
-        const int64_t ntiles = ceil_div(nxny, T::TL_Z);
-        const size_t shm = sizeof(cplx<R>) * (PL::P + zpass_lds_elems<PL, F, T::TL_Z, T::NCX_Z>());
-        constexpr int NT = plan_threads<PL, T::TL_Z, 2, 256>();
-        hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, T::TL_Z, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
+        const int64_t ntiles = ceil_div(nxny, TL);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + zpass_lds_elems<PL, F, TL, T::NCX_Z>());
+        constexpr int NT = plan_threads<PL, TL, 2, 256>();
+        hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
                            X, ldx, c2o, U, d, Pzs, tw, ntiles, s);
         BDDA_HIP(hipGetLastError());
     });
@@ -467,11 +513,12 @@ inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X,
     if (cells >= (1ull << 31) || cells * F * sizeof(cplx<R>) >= (1ull << 32)) return false;
     return with_plan(d.Pz, [&](auto pl) {
         using PL = decltype(pl);
+        constexpr int TL = plan_lines<PL, T::TL_Z, 2, 256>();
         const int64_t nxny = (int64_t)d.nx * d.ny;
-        const int64_t ntiles = ceil_div(nxny, T::TL_Z);
-        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)T::NCX_Z * T::TL_Z * PL::P);
-        constexpr int NT = plan_threads<PL, T::TL_Z, 2, 256>();
-        hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, T::TL_Z, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
+        const int64_t ntiles = ceil_div(nxny, TL);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)T::NCX_Z * TL * PL::P);
+        constexpr int NT = plan_threads<PL, TL, 2, 256>();
+        hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
                            U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles, s);
         BDDA_HIP(hipGetLastError());
     });
