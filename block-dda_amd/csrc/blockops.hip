@@ -268,34 +268,29 @@ __global__ __launch_bounds__(NT_RED) void k_gram_mfma(const cplx<R>* __restrict_
     }
 }
 
-// out[p] = sum_k partial[k np + p]; one 1024-thread group per 64 entries:
-// sixteen k-slices per entry, combined in LDS in a fixed order (deterministic)
-constexpr int RED_SLICES = 16;
-__global__ __launch_bounds__(64 * RED_SLICES) void k_reduce_partial(const c64* __restrict__ partial, int nblk, int np,
-                                                                    c64* __restrict__ out)
+// out[p] = sum_k partial[k np + p]: one wave per entry (four per group), the
+// lanes striding over k and then combined by a fixed butterfly of shuffles,
+// so the order of the additions never changes (deterministic).  The
+// partials were just written and are read from L2.
+constexpr int RED_WAVES = 4;
+__global__ __launch_bounds__(64 * RED_WAVES) void k_reduce_partial(const c64* __restrict__ partial, int nblk, int np,
+                                                                   c64* __restrict__ out)
 {
-    __shared__ double sh[2][RED_SLICES][64];
-    const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
-    const int p = blockIdx.x * 64 + lane;
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * RED_WAVES + (threadIdx.x >> 6);
+    if (p >= np) return;  // whole wave
     double sx = 0, sy = 0;
-    if (p < np)
-        for (int k = slice; k < nblk; k += RED_SLICES) {
-            const c64 v = partial[(int64_t)k * np + p];
-            sx += v.x;
-            sy += v.y;
-        }
-    sh[0][slice][lane] = sx;
-    sh[1][slice][lane] = sy;
-    __syncthreads();
-    if (slice == 0 && p < np) {
-        double tx = 0, ty = 0;
-#pragma unroll
-        for (int k = 0; k < RED_SLICES; ++k) {
-            tx += sh[0][k][lane];
-            ty += sh[1][k][lane];
-        }
-        out[p] = mk(tx, ty);
+    for (int k = lane; k < nblk; k += 64) {
+        const c64 v = partial[(int64_t)k * np + p];
+        sx += v.x;
+        sy += v.y;
     }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        sx += __shfl_xor(sx, m, 64);
+        sy += __shfl_xor(sy, m, 64);
+    }
+    if (lane == 0) out[p] = mk(sx, sy);
 }
 
 template <typename R, bool CONJ>
@@ -456,7 +451,7 @@ static bool force_valu_gram()
 // reduce the groups' partials and bring the a x b result to the host
 static HostMat finish_gram(BlockWork& w, int nblk, int np)
 {
-    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, 64)), dim3(64 * RED_SLICES), 0, w.stream,
+    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, RED_WAVES)), dim3(64 * RED_WAVES), 0, w.stream,
                        (const c64*)w.partial, nblk, np, w.red);
     BDDA_HIP(hipGetLastError());
     HostMat out(np);
@@ -526,7 +521,7 @@ c64 dot_all(BlockWork& w, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64
         hipLaunchKernelGGL((k_dot_partial<R, false>), dim3(nblk), dim3(NT_RED), 0, w.stream, A, lda, B, ldb, rows, s,
                            w.partial);
     BDDA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_partial, dim3(1), dim3(64 * RED_SLICES), 0, w.stream, (const c64*)w.partial, nblk, 1,
+    hipLaunchKernelGGL(k_reduce_partial, dim3(1), dim3(64 * RED_WAVES), 0, w.stream, (const c64*)w.partial, nblk, 1,
                        w.red);
     BDDA_HIP(hipGetLastError());
     BDDA_HIP(hipMemcpyAsync(w.host_pinned + w.host_elems, w.red, sizeof(c64), hipMemcpyDeviceToHost, w.stream));

@@ -188,6 +188,40 @@ struct PlanFFT {
         });
     }
 
+    // last-stage outputs in two phases: first key(j, RB, q) for every output
+    // position p = j + RB of the thread (a negative key skips it), then
+    // st(key, c, value).  A scatter whose addresses come from a lookup table
+    // issues all of its lookups before any dependent access, so their
+    // latencies overlap instead of adding up.  HALF as in store().
+    template <bool INV, bool HALF, class KEY, class ST>
+    __device__ __forceinline__ void store_keyed(KEY&& key, ST&& st)
+    {
+        constexpr int S = NS - 1, RAD = rad<S, INV>(), nb = P / RAD, total = nb * NQ, M = mb<S, INV>();
+        int kv[M][RAD];
+        static_for<0, M>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const int b = threadIdx.x + i * NT;
+            const bool ok = total % NT == 0 || b < total;
+            const int q = qof(b), j = jof(b);
+            static_for<0, RAD>([&](auto RR) {
+                constexpr int r = decltype(RR)::value;
+                if constexpr (!(HALF && r * nb >= (P + 1) / 2)) kv[i][r] = ok ? key(j, ic<r * nb>{}, q) : -1;
+            });
+        });
+        static_for<0, M>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            static_for<0, RAD>([&](auto RR) {
+                constexpr int r = decltype(RR)::value;
+                if constexpr (!(HALF && r * nb >= (P + 1) / 2)) {
+                    if (kv[i][r] >= 0) {
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) st(kv[i][r], c, v[c][i * RAD + r]);
+                    }
+                }
+            });
+        });
+    }
+
     // forward last-stage slots: fn(p, q, cplx<R>* vals[NC]) may rewrite vals
     template <class FN>
     __device__ __forceinline__ void visit(FN&& fn)

@@ -348,19 +348,37 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
     // H so that its index split is by a compile-time constant)
     const int PZ = d.nz | 1;
     constexpr int H = (P + 1) / 2;
+    // All occupancy lookups of a thread are issued before any of its X loads,
+    // and all X loads before the LDS stores, so the dependent loads overlap
+    // across the thread's cells instead of running one cell at a time.
     const int cell0 = (int)l0 * d.nz;
-    for (int e = threadIdx.x; e < TL * H; e += NT) {
-        const int p = e % H, tl = e / H;
-        if (p >= d.nz) continue;
-        int m = -1;
-        if (tl < nl) {
-            const int cell = cell0 + tl * d.nz + p;
-            m = c2o ? c2o[cell] : cell;
-        }
-        const uint32_t ob = (uint32_t)m * (uint32_t)(F * sizeof(cplx<R>));
+    constexpr int NIT = (TL * H + NT - 1) / NT;
+    int mi[NIT];
 #pragma unroll
-        for (int c = 0; c < F; ++c)
-            lds[(tl * F + c) * PZ + p] = (m >= 0) ? *at_b(Xr + c, ob) : czero<R>();
+    for (int it = 0; it < NIT; ++it) {
+        const int e = threadIdx.x + it * NT;
+        const int p = e % H, tl = e / H;
+        mi[it] = -1;
+        if (e < TL * H && p < d.nz && tl < nl) {
+            const int cell = cell0 + tl * d.nz + p;
+            mi[it] = c2o ? c2o[cell] : cell;
+        }
+    }
+    cplx<R> xv[NIT][F];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const uint32_t ob = (uint32_t)mi[it] * (uint32_t)(F * sizeof(cplx<R>));
+#pragma unroll
+        for (int c = 0; c < F; ++c) xv[it][c] = (mi[it] >= 0) ? *at_b(Xr + c, ob) : czero<R>();
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int e = threadIdx.x + it * NT;
+        const int p = e % H, tl = e / H;
+        if (e < TL * H && p < d.nz) {
+#pragma unroll
+            for (int c = 0; c < F; ++c) lds[(tl * F + c) * PZ + p] = xv[it][c];
+        }
     }
     __syncthreads();
     PlanFFT<R, PL, NT, TL, F, NCX> Z(tw, lds);
@@ -421,17 +439,20 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
     const cplx<R>* Xr = X + rhs * ldx;
     cplx<R>* Yr = Y + rhs * ldy;
     const int cell0 = (int)l0 * d.nz;
-    Z.template store<true, true>([&](int j, auto RB, int q, int c, cplx<R> v) {
-        const int p = j + decltype(RB)::value;
-        if (p >= d.nz || q >= nl) return;
-        const int cell = cell0 + q * d.nz + p;
-        const int m = c2o ? c2o[cell] : cell;
-        if (m < 0) return;
-        const uint32_t ob = (uint32_t)m * (uint32_t)(F * sizeof(cplx<R>));
-        if (diag) v = *at_b(diag + c, ob) * *at_b(Xr + c, ob) + v;
-        if (jpre) v = v * *at_b(jpre + c, ob);
-        *at_b(Yr + c, ob) = v;
-    });
+    // occupancy lookups of all the thread's cells first, then the updates
+    Z.template store_keyed<true, true>(
+        [&](int j, auto RB, int q) {
+            const int p = j + decltype(RB)::value;
+            if (p >= d.nz || q >= nl) return -1;
+            const int cell = cell0 + q * d.nz + p;
+            return c2o ? c2o[cell] : cell;
+        },
+        [&](int m, int c, cplx<R> v) {
+            const uint32_t ob = (uint32_t)m * (uint32_t)(F * sizeof(cplx<R>));
+            if (diag) v = *at_b(diag + c, ob) * *at_b(Xr + c, ob) + v;
+            if (jpre) v = v * *at_b(jpre + c, ob);
+            *at_b(Yr + c, ob) = v;
+        });
 }
 
 // ---------------------------------------------------------------- launchers
