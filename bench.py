@@ -57,6 +57,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+METHOD_NAMES = {"cocg_rq": "COCG-rq", "bicgstab": "BiCGSTAB", "bicggr": "BiCGGR"}
+
+
 def build_workload(name):
     W = importlib.import_module("block_dda_amd.workloads").build(name)
     log("workload: %s (%.1fs)" % (W.describe(), W.seconds))
@@ -223,7 +226,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="agg16384", choices=["agg16384", "agg1024", "agg256", "sphere16"])
+    ap.add_argument("--config", default="agg16384", choices=["agg16384", "agg1024", "agg4096", "agg256", "sphere16"])
+    ap.add_argument("--method", default="cocg_rq", choices=["cocg_rq", "bicgstab", "bicggr"],
+                    help="block solver (Jacobi variant); the wavelength-sweep workload agg4096 is solved "
+                         "with bicggr in its parity tests")
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--itermax", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -244,6 +250,7 @@ def main():
               "that is not run here).", file=sys.stderr)
         sys.exit(2)
     pkg = load_pkg()
+    method = {"cocg_rq": pkg.COCG_RQ, "bicgstab": pkg.BICGSTAB, "bicggr": pkg.BICGGR}[args.method]
     W = build_workload(args.config)
     tgt, k, diag, B = W.target, W.k, W.diag, W.B
     s = B.shape[1]
@@ -293,19 +300,19 @@ def main():
     fp64 = None
     obs64 = None
     if not args.no_fp64:
-        fp64 = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_FP64)
+        fp64 = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_FP64, method)
         fp64["true_res_col_max"] = residual_col_max(pkg, op, dB, dX, s)
         log("fp64 solve: %s" % fp64)
         obs64 = observables_dev(pkg, op, dB, dX, W.waves, s)
     # mixed precision (true FP64 residual <= tol): warmup + timed steps
     for _ in range(args.warmup):
-        r = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_MIXED)
+        r = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_MIXED, method)
         log("mixed warmup: %s" % r)
     runs = []
     pkg._check(L.bdda_device_sync())
     t_all = time.perf_counter()
     for _ in range(args.steps):
-        runs.append(solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_MIXED))
+        runs.append(solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_MIXED, method))
         log("mixed step: %s" % runs[-1])
     pkg._check(L.bdda_device_sync())
     total = time.perf_counter() - t_all
@@ -339,7 +346,8 @@ def main():
         "metric": METRIC, "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "complex64 inner + complex128 outer", "data": "synthetic (seeded aggregate, plane waves)",
-        "config": {"workload": "%s -> padded %s, block COCG-rq (Jacobi), tol %g" % (W.describe(), list(P), args.tol),
+        "config": {"workload": "%s -> padded %s, block %s (Jacobi), tol %g" % (
+            W.describe(), list(P), METHOD_NAMES[args.method], args.tol),
                    "parallelism": "single GPU"},
         "time_to_solution_with_init_s": tts + init_s,
         "mixed": {"iters": runs[-1]["iters"], "outer": runs[-1]["outer_iters"], "matvecs": runs[-1]["matvecs"],
