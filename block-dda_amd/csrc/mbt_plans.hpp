@@ -211,7 +211,6 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
 }
 
 // ---------------------------------------------------------------- x pass
-// A group owns (rhs, ky, NQ consecutive kz) across all F component planes.
 // Minimum waves per SIMD asked of the compiler.  FP32 plans whose butterfly
 // registers take at most 42 VGPRs (343 = 7^3: one butterfly of 7 values x 3
 // components per thread) fit in 72 VGPRs, so seven waves fit per SIMD (four
@@ -222,10 +221,11 @@ constexpr int xfused_min_waves()
     return (sizeof(R) == 4 && PlanFFT<R, PL, NT, NQ, F, 1>::E * F * 2 <= 42) ? 7 : 1;
 }
 
+// A group owns (rhs, ky, NQ consecutive kz) across all F component planes.
 template <typename R, class PL, int F, int NT, int NQ>
-__global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_xfused_plan(cplx<R>* __restrict__ V, const cplx<R>* __restrict__ Gh,
-                                                    MbtDims d, int Pzs, const cplx<R>* __restrict__ twg, int ntz,
-                                                    int nrhs)
+__global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_xfused_plan(
+    cplx<R>* __restrict__ V, const cplx<R>* __restrict__ Gh, MbtDims d, int Pzs, const cplx<R>* __restrict__ twg,
+    int ntz, int nrhs)
 {
     constexpr int P = PL::P;
     constexpr int NC = (F == 3) ? 6 : 9;
