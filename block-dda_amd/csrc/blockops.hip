@@ -373,13 +373,20 @@ __global__ __launch_bounds__(256) void k_combo(ComboDev<R> c, cplx<R>* Z, int64_
                 if (u < c.nm) {
                     const cplx<R>* W = c.W[u] + r;
                     const int kw = c.kw[u];
-                    cplx<R> w[KW];
+                    // W in chunks of 16 columns: a wide block is read once per
+                    // launch without holding all of its row in registers
+                    constexpr int KC = KW < 16 ? KW : 16;
+#pragma unroll 1
+                    for (int l0 = 0; l0 < KW; l0 += KC) {
+                        cplx<R> w[KC];
 #pragma unroll
-                    for (int l = 0; l < KW; ++l) w[l] = (l < kw) ? W[(int64_t)l * c.ldw[u]] : czero<R>();
+                        for (int l = 0; l < KC; ++l)
+                            w[l] = (l0 + l < kw) ? W[(int64_t)(l0 + l) * c.ldw[u]] : czero<R>();
 #pragma unroll
-                    for (int l = 0; l < KW; ++l)
+                        for (int l = 0; l < KC; ++l)
 #pragma unroll
-                        for (int jj = 0; jj < SJ; ++jj) fma_c(acc[jj], w[l], Ms[u][l][jj]);
+                            for (int jj = 0; jj < SJ; ++jj) fma_c(acc[jj], w[l], Ms[u][l0 + l][jj]);
+                    }
                 }
         }
 #pragma unroll
@@ -537,12 +544,12 @@ double norm_fro(BlockWork& w, const cplx<R>* A, int64_t lda, int64_t rows, int s
 }
 
 
-// column chunk width: wide coefficient blocks get narrower chunks so that
-// a row's KW inputs and SJ accumulators stay in registers
+// column chunk width: sixteen output columns per launch; the W block is
+// read in 16-column pieces inside the kernel
 template <typename R, int KW>
 static void launch_combo(hipStream_t st, const ComboDev<R>& cd, cplx<R>* Z, int64_t ldz, int64_t rows, int s)
 {
-    constexpr int SJMAX = KW <= 16 ? 16 : (KW == 32 ? 8 : 4);
+    constexpr int SJMAX = 16;
     int sj = SJMAX;
     while (sj > 1 && sj / 2 >= s) sj /= 2;
     const dim3 grid((unsigned)std::max<int64_t>(1, ceil_div(rows, 256)));
@@ -822,35 +829,79 @@ int qr_fast(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq
 }
 
 // ------------------------------------------------------------- host algebra
-HostMat hm_mul(const HostMat& A, const HostMat& B, int m, int k, int n)
+// The s x s algebra below runs on split real / imaginary planes, column by
+// column over contiguous rows, so the compiler vectorises the inner loops
+// (the interleaved complex form does not vectorise); the hot routines are
+// also built for AVX2 + FMA, used where the CPU has them.  At s = 64 the
+// solver loops spend milliseconds per iteration here otherwise.
+#define BDDA_INLINE inline __attribute__((always_inline))
+static bool host_avx2()
 {
-    HostMat C((size_t)m * n, mk(0.0, 0.0));
-    for (int j = 0; j < n; ++j)
+    static const bool v = __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+    return v;
+}
+namespace {
+struct Planes {
+    std::vector<double> re, im;
+    explicit Planes(const HostMat& A) : re(A.size()), im(A.size())
+    {
+        for (size_t e = 0; e < A.size(); ++e) {
+            re[e] = A[e].x;
+            im[e] = A[e].y;
+        }
+    }
+    Planes(size_t n, double v) : re(n, v), im(n, 0.0) {}
+    HostMat out() const
+    {
+        HostMat C(re.size());
+        for (size_t e = 0; e < re.size(); ++e) C[e] = mk(re[e], im[e]);
+        return C;
+    }
+};
+}  // namespace
+
+// C (m x n) += A (m x k, planes) B (k x n)
+static BDDA_INLINE void mul_body(const double* __restrict__ Ar, const double* __restrict__ Ai, const c64* B,
+                                 double* __restrict__ Cr, double* __restrict__ Ci, int m, int k, int n)
+{
+    for (int j = 0; j < n; ++j) {
+        double* __restrict__ cr = Cr + (size_t)j * m;
+        double* __restrict__ ci = Ci + (size_t)j * m;
         for (int l = 0; l < k; ++l) {
             const c64 b = B[l + (size_t)j * k];
+            const double* __restrict__ ar = Ar + (size_t)l * m;
+            const double* __restrict__ ai = Ai + (size_t)l * m;
             for (int i = 0; i < m; ++i) {
-                const c64 a = A[i + (size_t)l * m];
-                C[i + (size_t)j * m].x += a.x * b.x - a.y * b.y;
-                C[i + (size_t)j * m].y += a.x * b.y + a.y * b.x;
+                cr[i] += ar[i] * b.x - ai[i] * b.y;
+                ci[i] += ar[i] * b.y + ai[i] * b.x;
             }
         }
-    return C;
+    }
+}
+__attribute__((target("avx2,fma"))) static void mul_avx2(const double* Ar, const double* Ai, const c64* B, double* Cr,
+                                                           double* Ci, int m, int k, int n)
+{
+    mul_body(Ar, Ai, B, Cr, Ci, m, k, n);
+}
+static void mul_base(const double* Ar, const double* Ai, const c64* B, double* Cr, double* Ci, int m, int k, int n)
+{
+    mul_body(Ar, Ai, B, Cr, Ci, m, k, n);
+}
+
+HostMat hm_mul(const HostMat& A, const HostMat& B, int m, int k, int n)
+{
+    const Planes a(A);
+    Planes c((size_t)m * n, 0.0);
+    (host_avx2() ? mul_avx2 : mul_base)(a.re.data(), a.im.data(), B.data(), c.re.data(), c.im.data(), m, k, n);
+    return c.out();
 }
 
 HostMat hm_tmul(const HostMat& A, const HostMat& B, int k, int m, int n)
 {
-    HostMat C((size_t)m * n, mk(0.0, 0.0));
-    for (int j = 0; j < n; ++j)
-        for (int i = 0; i < m; ++i) {
-            c64 s = mk(0.0, 0.0);
-            for (int l = 0; l < k; ++l) {
-                const c64 a = A[l + (size_t)i * k], b = B[l + (size_t)j * k];
-                s.x += a.x * b.x - a.y * b.y;
-                s.y += a.x * b.y + a.y * b.x;
-            }
-            C[i + (size_t)j * m] = s;
-        }
-    return C;
+    HostMat At((size_t)m * k);  // A is k x m; A^T as an m x k column-major matrix
+    for (int l = 0; l < k; ++l)
+        for (int i = 0; i < m; ++i) At[i + (size_t)l * m] = A[l + (size_t)i * k];
+    return hm_mul(At, B, m, k, n);
 }
 
 double hm_fro(const HostMat& A)
@@ -860,65 +911,105 @@ double hm_fro(const HostMat& A)
     return std::sqrt(s);
 }
 
-bool hm_lu_solve(const HostMat& A_, const HostMat& B, int s, int nb, int full, HostMat& X)
+// LU with partial (full = 0) or full (full = 1) pivoting of the planes
+// Ar, Ai in place, then X = A^-1 B; rp, cp, yr, yi are s-long scratch
+static BDDA_INLINE bool lu_body(double* __restrict__ Ar, double* __restrict__ Ai, const c64* B, int s, int nb,
+                                int full, c64* X, int* rp, int* cp, double* yr, double* yi)
 {
-    HostMat A = A_;
-    std::vector<int> rp(s), cp(s);
     for (int i = 0; i < s; ++i) rp[i] = cp[i] = i;
-    auto abs2 = [](c64 v) { return std::sqrt(v.x * v.x + v.y * v.y); };
     for (int k = 0; k < s; ++k) {
         int pr = k, pc = k;
         double best = -1;
         for (int j = k; j < (full ? s : k + 1); ++j)
             for (int i = k; i < s; ++i) {
-                const double v = abs2(A[i + (size_t)j * s]);
+                const size_t e = i + (size_t)j * s;
+                const double v = std::sqrt(Ar[e] * Ar[e] + Ai[e] * Ai[e]);
                 if (v > best) { best = v; pr = i; pc = j; }
             }
         if (pr != k) {
-            for (int j = 0; j < s; ++j) std::swap(A[k + (size_t)j * s], A[pr + (size_t)j * s]);
+            for (int j = 0; j < s; ++j) {
+                std::swap(Ar[k + (size_t)j * s], Ar[pr + (size_t)j * s]);
+                std::swap(Ai[k + (size_t)j * s], Ai[pr + (size_t)j * s]);
+            }
             std::swap(rp[k], rp[pr]);
         }
         if (pc != k) {
-            for (int i = 0; i < s; ++i) std::swap(A[i + (size_t)k * s], A[i + (size_t)pc * s]);
+            for (int i = 0; i < s; ++i) {
+                std::swap(Ar[i + (size_t)k * s], Ar[i + (size_t)pc * s]);
+                std::swap(Ai[i + (size_t)k * s], Ai[i + (size_t)pc * s]);
+            }
             std::swap(cp[k], cp[pc]);
         }
-        const c64 piv = A[k + (size_t)k * s];
-        const double den = piv.x * piv.x + piv.y * piv.y;
+        const double pxr = Ar[k + (size_t)k * s], pxi = Ai[k + (size_t)k * s];
+        const double den = pxr * pxr + pxi * pxi;
         if (!(den > 0)) return false;
-        for (int i = k + 1; i < s; ++i) {
-            const c64 a = A[i + (size_t)k * s];
-            const c64 l = mk((a.x * piv.x + a.y * piv.y) / den, (a.y * piv.x - a.x * piv.y) / den);
-            A[i + (size_t)k * s] = l;
-            for (int j = k + 1; j < s; ++j) {
-                const c64 u = A[k + (size_t)j * s];
-                A[i + (size_t)j * s].x -= l.x * u.x - l.y * u.y;
-                A[i + (size_t)j * s].y -= l.x * u.y + l.y * u.x;
+        double* __restrict__ lr = &Ar[(size_t)k * s];
+        double* __restrict__ li = &Ai[(size_t)k * s];
+        for (int i = k + 1; i < s; ++i) {  // multipliers l_i = a_ik / pivot, kept in column k
+            const double xr = lr[i], xi = li[i];
+            lr[i] = (xr * pxr + xi * pxi) / den;
+            li[i] = (xi * pxr - xr * pxi) / den;
+        }
+        for (int j = k + 1; j < s; ++j) {  // trailing columns, contiguous rows
+            const double ur = Ar[k + (size_t)j * s], ui = Ai[k + (size_t)j * s];
+            double* __restrict__ cr = &Ar[(size_t)j * s];
+            double* __restrict__ ci = &Ai[(size_t)j * s];
+            for (int i = k + 1; i < s; ++i) {
+                cr[i] -= lr[i] * ur - li[i] * ui;
+                ci[i] -= lr[i] * ui + li[i] * ur;
             }
         }
     }
-    X.assign((size_t)s * nb, mk(0.0, 0.0));
-    std::vector<c64> y(s);
     for (int c = 0; c < nb; ++c) {
-        for (int i = 0; i < s; ++i) y[i] = B[rp[i] + (size_t)c * s];
-        for (int i = 0; i < s; ++i)
-            for (int j = 0; j < i; ++j) {
-                const c64 a = A[i + (size_t)j * s];
-                y[i].x -= a.x * y[j].x - a.y * y[j].y;
-                y[i].y -= a.x * y[j].y + a.y * y[j].x;
-            }
-        for (int i = s - 1; i >= 0; --i) {
-            for (int j = i + 1; j < s; ++j) {
-                const c64 a = A[i + (size_t)j * s];
-                y[i].x -= a.x * y[j].x - a.y * y[j].y;
-                y[i].y -= a.x * y[j].y + a.y * y[j].x;
-            }
-            const c64 d = A[i + (size_t)i * s];
-            const double den = d.x * d.x + d.y * d.y;
-            y[i] = mk((y[i].x * d.x + y[i].y * d.y) / den, (y[i].y * d.x - y[i].x * d.y) / den);
+        for (int i = 0; i < s; ++i) {
+            yr[i] = B[rp[i] + (size_t)c * s].x;
+            yi[i] = B[rp[i] + (size_t)c * s].y;
         }
-        for (int i = 0; i < s; ++i) X[cp[i] + (size_t)c * s] = y[i];
+        for (int j = 0; j < s; ++j) {  // unit lower triangle, by columns
+            const double vr = yr[j], vi = yi[j];
+            const double* __restrict__ cr = &Ar[(size_t)j * s];
+            const double* __restrict__ ci = &Ai[(size_t)j * s];
+            for (int i = j + 1; i < s; ++i) {
+                yr[i] -= cr[i] * vr - ci[i] * vi;
+                yi[i] -= cr[i] * vi + ci[i] * vr;
+            }
+        }
+        for (int j = s - 1; j >= 0; --j) {  // upper triangle, by columns
+            const double dr = Ar[j + (size_t)j * s], di = Ai[j + (size_t)j * s];
+            const double dd = dr * dr + di * di;
+            const double vr = (yr[j] * dr + yi[j] * di) / dd, vi = (yi[j] * dr - yr[j] * di) / dd;
+            yr[j] = vr;
+            yi[j] = vi;
+            const double* __restrict__ cr = &Ar[(size_t)j * s];
+            const double* __restrict__ ci = &Ai[(size_t)j * s];
+            for (int i = 0; i < j; ++i) {
+                yr[i] -= cr[i] * vr - ci[i] * vi;
+                yi[i] -= cr[i] * vi + ci[i] * vr;
+            }
+        }
+        for (int i = 0; i < s; ++i) X[cp[i] + (size_t)c * s] = mk(yr[i], yi[i]);
     }
     return true;
+}
+__attribute__((target("avx2,fma"))) static bool lu_avx2(double* Ar, double* Ai, const c64* B, int s, int nb, int full,
+                                                         c64* X, int* rp, int* cp, double* yr, double* yi)
+{
+    return lu_body(Ar, Ai, B, s, nb, full, X, rp, cp, yr, yi);
+}
+static bool lu_base(double* Ar, double* Ai, const c64* B, int s, int nb, int full, c64* X, int* rp, int* cp,
+                    double* yr, double* yi)
+{
+    return lu_body(Ar, Ai, B, s, nb, full, X, rp, cp, yr, yi);
+}
+
+bool hm_lu_solve(const HostMat& A_, const HostMat& B, int s, int nb, int full, HostMat& X)
+{
+    Planes a(A_);
+    std::vector<int> rp(s), cp(s);
+    std::vector<double> yr(s), yi(s);
+    X.assign((size_t)s * nb, mk(0.0, 0.0));
+    return (host_avx2() ? lu_avx2 : lu_base)(a.re.data(), a.im.data(), B.data(), s, nb, full, X.data(), rp.data(),
+                                              cp.data(), yr.data(), yi.data());
 }
 
 // ------------------------------------------------------------- instantiations
