@@ -81,6 +81,29 @@ def test_mixed_precision_reaches_fp64_accuracy(bdda, orc, method):
     assert np.linalg.norm(X - Xd) / np.linalg.norm(Xd) < 1e-7
 
 
+@pytest.mark.parametrize("L", [24, 40, 64])
+def test_wide_blocks_against_dense(bdda, orc, L):
+    """Blocks wider than 16 columns (Gram products and block updates on their
+    wide paths, widths not a multiple of 16): FP64 and mixed solves against
+    the dense direct solve."""
+    print("case wide", L, flush=True)
+    n, addr, diag, B, lf, k = problem(bdda, 3, n=(8, 8, 8), nocc=400, seed=8, L=L, two_materials=False)
+    op = bdda.MBT_fft_init(n, 3, lf, k)
+    op.set_target(addr, diag)
+    A = orc.dense_system(n, 3, lf, k, addr, diag)
+    Xd = np.linalg.solve(A, B)
+    for mode, tol in ((bdda.MODE_FP64, 1e-9), (bdda.MODE_MIXED, 1e-8)):
+        X, info, _ = op.solve(B, tol, 600, method=bdda.COCG_RQ, jacobi=True, mode=mode)
+        assert info["converged"] == 1
+        res = np.linalg.norm(A @ X - B) / np.linalg.norm(B)
+        assert res <= 50 * tol, (mode, res)
+        assert np.linalg.norm(X - Xd) / np.linalg.norm(Xd) < 1e-5
+    for meth in (bdda.BICGSTAB, bdda.BICGGR):
+        X, info, _ = op.solve(B, 1e-9, 600, method=meth, jacobi=True, mode=bdda.MODE_FP64)
+        assert info["converged"] == 1
+        assert np.linalg.norm(A @ X - B) / np.linalg.norm(B) <= 50e-9
+
+
 def test_residual_columns_against_dense(bdda, orc):
     """Per-column true residuals: an FP32 solve (columns converge unevenly)
     and an arbitrary X with a zero column of B (absolute residual there)."""
