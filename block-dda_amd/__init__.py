@@ -256,7 +256,7 @@ class MbtOperator:
 
     def solve(self, B, tol, itermax, method=COCG_RQ, jacobi=False, mode=MODE_FP64, hist_len=0):
         B = _cz(B)
-        X = np.zeros_like(B, order="F")
+        X = np.empty_like(B, order="F")  # every entry is written (also when itermax is reached)
         info = SolveInfo()
         hist = np.zeros(max(hist_len, 1))
         rc = lib().bdda_solve(self.h, int(method), int(jacobi), int(mode), _ptr(B), B.shape[0], B.shape[1],

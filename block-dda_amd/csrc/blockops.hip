@@ -268,6 +268,67 @@ __global__ __launch_bounds__(NT_RED) void k_gram_mfma(const cplx<R>* __restrict_
     }
 }
 
+// Wider blocks (a, b <= 16 NW, NW = 2 or 4) on the matrix cores: the output
+// is NW x NW tiles of 16 x 16, dealt to the four waves (a wave keeps its
+// tiles' accumulators over all of the group's rows, so the waves need no
+// final reduction).  Row tiles of MF_TRW rows x 16 NW columns sit in LDS;
+// the operand layouts are those of k_gram_mfma.
+constexpr int MF_TRW = 32;
+template <typename R, bool CONJ, bool SYM, int NW>
+__global__ __launch_bounds__(NT_RED) void k_gram_mfma_wide(const cplx<R>* __restrict__ A, int64_t lda, int a,
+                                                          const cplx<R>* __restrict__ B, int64_t ldb, int b,
+                                                          int64_t rows, int64_t rows_per_blk, c64* __restrict__ partial)
+{
+    constexpr int WC = 16 * NW, PITCH = WC + 1;
+    constexpr int TPW = (NW * NW + 3) / 4;  // tiles per wave
+    __shared__ __align__(16) cplx<R> tiles[(SYM ? 1 : 2) * MF_TRW * PITCH];
+    cplx<R>* As = tiles;
+    cplx<R>* Bs = SYM ? As : As + MF_TRW * PITCH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    MfmaAcc<R> acc[TPW];
+    const int64_t rb = blockIdx.x * rows_per_blk;
+    const int64_t re = min(rows, rb + rows_per_blk);
+    for (int64_t r0 = rb; r0 < re; r0 += MF_TRW) {
+        const int nr = (int)min((int64_t)MF_TRW, re - r0);
+        for (int e = tid; e < MF_TRW * WC; e += NT_RED) {
+            const int r = e % MF_TRW, col = e / MF_TRW;
+            As[r * PITCH + col] = (r < nr && col < a) ? A[col * lda + r0 + r] : czero<R>();
+            if constexpr (!SYM) Bs[r * PITCH + col] = (r < nr && col < b) ? B[col * ldb + r0 + r] : czero<R>();
+        }
+        __syncthreads();
+        const int col = lane & 15;
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+            const int t = wave + 4 * u;
+            if (t < NW * NW) {
+                const int ta = t / NW, tb = t % NW;
+#pragma unroll
+                for (int st = 0; st < MF_TRW / 4; ++st) {
+                    const int row = 4 * st + (lane >> 4);
+                    const cplx<R> x = As[row * PITCH + ta * 16 + col];
+                    const cplx<R> y = Bs[row * PITCH + tb * 16 + col];
+                    acc[u].step(x.x, x.y, y.x, y.y, CONJ);
+                }
+                acc[u].flush();
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        const int t = wave + 4 * u;
+        if (t < NW * NW) {
+            const int ta = t / NW, tb = t % NW;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = ta * 16 + MfmaAcc<R>::row(lane, v), jj = tb * 16 + (lane & 15);
+                if (i < a && jj < b)
+                    partial[(int64_t)blockIdx.x * a * b + i + (int64_t)jj * a] = mk(acc[u].get_re(v), acc[u].get_im(v));
+            }
+        }
+    }
+}
+
 // out[p] = sum_k partial[k np + p]: one wave per entry (four per group), the
 // lanes striding over k and then combined by a fixed butterfly of shuffles,
 // so the order of the additions never changes (deterministic).  The
@@ -395,6 +456,27 @@ __global__ __launch_bounds__(256) void k_combo(ComboDev<R> c, cplx<R>* Z, int64_
     }
 }
 
+// per group: the number of non-finite real or imaginary parts among its
+// grid-stride share of n values (the partial sums of all_finite)
+__global__ __launch_bounds__(NT_RED) void k_bad_partial(const c64* __restrict__ A, int64_t n, c64* __restrict__ partial)
+{
+    __shared__ double sh[NT_RED / 64];
+    double bad = 0;
+    for (int64_t e = blockIdx.x * (int64_t)NT_RED + threadIdx.x; e < n; e += (int64_t)gridDim.x * NT_RED) {
+        const c64 v = A[e];
+        bad += (std::isfinite(v.x) ? 0.0 : 1.0) + (std::isfinite(v.y) ? 0.0 : 1.0);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) bad += __shfl_xor(bad, m, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = bad;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < NT_RED / 64; ++k) t += sh[k];
+        partial[blockIdx.x] = mk(t, 0.0);
+    }
+}
+
 template <typename R>
 __global__ void k_scale_rows(const cplx<R>* __restrict__ d, const cplx<R>* A, int64_t lda, cplx<R>* Z, int64_t ldz,
                              int64_t rows)
@@ -496,6 +578,36 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
         BDDA_HIP(hipGetLastError());
         return finish_gram(w, nblk, np);
     }
+    if (a <= 64 && b <= 64 && !force_valu_gram()) {
+        // wide blocks on the matrix cores; at most 256 groups keep the
+        // partials (a b entries each) small
+        const int nw = (a <= 32 && b <= 32) ? 2 : 4;
+        int64_t rpb = std::max<int64_t>(MF_TRW, ceil_div(std::max<int64_t>(rows, 1), 256));
+        rpb = ceil_div(rpb, MF_TRW) * MF_TRW;
+        const int nblk = (int)std::max<int64_t>(1, ceil_div(rows, rpb));
+        w.reserve((size_t)nblk * np, np);
+        const bool sym = (const void*)A == (const void*)B && lda == ldb && a == b;
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(nblk), dim3(NT_RED), 0, w.stream, A, lda, a, B, ldb, b, rows, rpb,
+                               w.partial);
+        };
+#define BDDA_GRAM_WIDE(NW)                                                      \
+    if (conjA) {                                                                \
+        if (sym) launch(k_gram_mfma_wide<R, true, true, NW>);                   \
+        else launch(k_gram_mfma_wide<R, true, false, NW>);                      \
+    } else {                                                                    \
+        if (sym) launch(k_gram_mfma_wide<R, false, true, NW>);                  \
+        else launch(k_gram_mfma_wide<R, false, false, NW>);                     \
+    }
+        if (nw == 2) {
+            BDDA_GRAM_WIDE(2)
+        } else {
+            BDDA_GRAM_WIDE(4)
+        }
+#undef BDDA_GRAM_WIDE
+        BDDA_HIP(hipGetLastError());
+        return finish_gram(w, nblk, np);
+    }
     const size_t c = sizeof(cplx<R>);
     const int pitch = 4 * ((a + 3) / 4) + 4 * ((b + 3) / 4) + 2;
     int TR = (int)std::min<int64_t>(128, 32768 / (pitch * c));
@@ -541,6 +653,20 @@ double norm_fro(BlockWork& w, const cplx<R>* A, int64_t lda, int64_t rows, int s
 {
     const c64 v = dot_all<R>(w, A, lda, A, lda, rows, s, 1);
     return std::sqrt(std::max(v.x, 0.0));
+}
+
+bool all_finite(BlockWork& w, const c64* A, int64_t n)
+{
+    const int nblk = (int)std::min<int64_t>(MAX_PARTIAL_BLOCKS, std::max<int64_t>(1, ceil_div(n, NT_RED * 8)));
+    w.reserve(nblk, 1);
+    hipLaunchKernelGGL(k_bad_partial, dim3(nblk), dim3(NT_RED), 0, w.stream, A, n, w.partial);
+    BDDA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partial, dim3(1), dim3(64 * RED_WAVES), 0, w.stream, (const c64*)w.partial, nblk, 1,
+                       w.red);
+    BDDA_HIP(hipGetLastError());
+    BDDA_HIP(hipMemcpyAsync(w.host_pinned + w.host_elems, w.red, sizeof(c64), hipMemcpyDeviceToHost, w.stream));
+    BDDA_HIP(hipStreamSynchronize(w.stream));
+    return w.host_pinned[w.host_elems].x == 0.0;
 }
 
 

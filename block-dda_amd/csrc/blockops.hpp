@@ -41,6 +41,9 @@ c64 dot_all(BlockWork& w, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64
 template <typename R>
 double norm_fro(BlockWork& w, const cplx<R>* A, int64_t lda, int64_t rows, int s);
 
+// no NaN or Inf among the n values of A (device, FP64 complex)
+bool all_finite(BlockWork& w, const c64* A, int64_t n);
+
 // Z = sum_t a_t V_t + sum_u W_u M_u  (M_u: host FP64, k_u x s, column-major).
 // Z may alias a V_t but never a W_u.
 template <typename R>

@@ -99,6 +99,11 @@ static int check_prec(const bdda_mbt* h, int prec)
 template <typename R>
 static void to_dev(const bdda_c64* src, int64_t ld, int64_t rows, int cols, cplx<R>* dst, hipStream_t st)
 {
+    if (sizeof(R) == sizeof(double) && ld == rows) {  // same layout: one copy, no staging vector
+        BDDA_HIP(hipMemcpyAsync(dst, src, sizeof(cplx<R>) * rows * cols, hipMemcpyHostToDevice, st));
+        BDDA_HIP(hipStreamSynchronize(st));
+        return;
+    }
     std::vector<cplx<R>> tmp((size_t)rows * cols);
     for (int j = 0; j < cols; ++j)
         for (int64_t i = 0; i < rows; ++i) tmp[(size_t)j * rows + i] = mk((R)src[j * ld + i].re, (R)src[j * ld + i].im);
@@ -109,6 +114,11 @@ static void to_dev(const bdda_c64* src, int64_t ld, int64_t rows, int cols, cplx
 template <typename R>
 static void from_dev(const cplx<R>* src, int64_t rows, int cols, bdda_c64* dst, int64_t ld, hipStream_t st)
 {
+    if (sizeof(R) == sizeof(double) && ld == rows) {
+        BDDA_HIP(hipMemcpyAsync(dst, src, sizeof(cplx<R>) * rows * cols, hipMemcpyDeviceToHost, st));
+        BDDA_HIP(hipStreamSynchronize(st));
+        return;
+    }
     std::vector<cplx<R>> tmp((size_t)rows * cols);
     BDDA_HIP(hipMemcpyAsync(tmp.data(), src, sizeof(cplx<R>) * tmp.size(), hipMemcpyDeviceToHost, st));
     BDDA_HIP(hipStreamSynchronize(st));
@@ -149,6 +159,7 @@ static int solve_common(bdda_mbt* h, int method, int jacobi, int mode, const c64
     if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
     if ((mode != MODE_FP64 && !h->op->have32) || (mode != MODE_FP32 && !h->op->have64))
         return fail(BDDA_EINVAL, "operator lacks the precision this mode needs");
+    if (!all_finite(h->prob.work, dB, h->prob.rows() * L)) return fail(BDDA_ENAN, "NaN or Inf in B");
     SolveInfo si;
     std::vector<double> hv;
     int rc = solve_fp(h->prob, method, jacobi, mode, dB, dX, L, tol, itermax, si, hist ? &hv : nullptr, 0.0);
@@ -397,10 +408,7 @@ int bdda_solve(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64* B,
     if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
     const int64_t rows = h->prob.rows();
     if (ldb < rows || ldx < rows) return fail(BDDA_EINVAL, "leading dimension smaller than f*nocc");
-    for (int j = 0; j < L; ++j)
-        for (int64_t i = 0; i < rows; ++i)
-            if (!std::isfinite(B[j * ldb + i].re) || !std::isfinite(B[j * ldb + i].im))
-                return fail(BDDA_ENAN, "NaN or Inf in B");
+    if (L < 1 || L > 64) return fail(BDDA_EINVAL, "number of right-hand sides must be in 1..64");
     BDDA_TRY({
         c64* dB = nullptr;
         c64* dX = nullptr;

@@ -170,6 +170,10 @@ def test_solver_error_paths(bdda, orc):
     with pytest.raises(bdda.BddaError) as e:
         op.solve(Bn, 1e-6, 50)
     assert e.value.code == 6
+    Bn[5, 1] = complex(1.0, np.inf)  # the check is made on the device copy of B
+    with pytest.raises(bdda.BddaError) as e:
+        op.solve(Bn, 1e-6, 50)
+    assert e.value.code == 6 and "B" in str(e.value)
     Bd = B.copy()
     Bd[:, 2] = Bd[:, 0]          # duplicate right-hand side: rank-deficient block
     with pytest.raises(bdda.BddaError) as e:
