@@ -66,10 +66,16 @@ def build_workload(name):
     return W
 
 
-def pass_bytes(n, P, nocc, s, f, c, nc):
+def occupied_lines(n, address):
+    """(ix, iy) lines that hold an occupied cell: the z passes and the y passes'
+    short side move only their rows."""
+    return int(np.unique(np.asarray(address, dtype=np.int64) // n[2]).size)
+
+
+def pass_bytes(n, P, nocc, s, f, c, nc, nlines=None):
     nx, ny, nz = n
     Px, Py, Pz = P
-    nxnyPz = nx * ny * Pz
+    nxnyPz = (nx * ny if nlines is None else nlines) * Pz
     nxPyPz = nx * Py * Pz
     gh = nc * (Px // 2 + 1) * (Py // 2 + 1) * (Pz // 2 + 1) * c
     return {
@@ -274,7 +280,7 @@ def main():
     mv = {}
     for name, prec, c in (("fp32", pkg.BDDA_PREC_FP32, 8), ("fp64", pkg.BDDA_PREC_FP64, 16)):
         ms, wall = time_matvec(pkg, op, rows, s, prec, args.matvec_reps)
-        by = pass_bytes(tgt.n, P, tgt.nocc, s, 3, c, 6)
+        by = pass_bytes(tgt.n, P, tgt.nocc, s, 3, c, 6, occupied_lines(tgt.n, tgt.address))
         mv[name] = {"pass_ms": ms, "wall_ms": wall,
                     "pass_GBs": {kk: by[kk] / (ms[kk] * 1e-3) / 1e9 for kk in ms},
                     "matvec_GBs": sum(by.values()) / (sum(ms.values()) * 1e-3) / 1e9,

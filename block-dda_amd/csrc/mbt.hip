@@ -251,6 +251,8 @@ void mbt_destroy(MbtOperator* op)
     if (op->ws) (void)hipFree(op->ws);
     if (op->c2o) (void)hipFree(op->c2o);
     if (op->addr_dev) (void)hipFree(op->addr_dev);
+    if (op->lines) (void)hipFree(op->lines);
+    if (op->lmask) (void)hipFree(op->lmask);
     for (auto& e : op->ev)
         if (e) (void)hipEventDestroy(e);
     delete op;
@@ -275,7 +277,23 @@ void mbt_set_address(MbtOperator* op, const int* address_host, int64_t nocc)
     hipLaunchKernelGGL(k_build_c2o, dim3(grid_for(nocc)), dim3(256), 0, op->stream, (const int*)op->addr_dev, nocc,
                        op->c2o);
     BDDA_HIP(hipGetLastError());
+    // occupied (ix, iy) lines: flags and the ascending list of their ids
+    const int64_t nxny = (int64_t)op->d.nx * op->d.ny;
+    std::vector<unsigned char> mask(nxny, 0);
+    for (int64_t m = 0; m < nocc; ++m) mask[address_host[m] / op->d.nz] = 1;
+    std::vector<int> ids;
+    for (int64_t l = 0; l < nxny; ++l)
+        if (mask[l]) ids.push_back((int)l);
+    if (!op->lmask) BDDA_HIP(hipMalloc(&op->lmask, nxny));
+    if (op->lines) BDDA_HIP(hipFree(op->lines));
+    op->lines = nullptr;
+    op->nlines = 0;
+    BDDA_HIP(hipMalloc(&op->lines, sizeof(int) * std::max<size_t>(ids.size(), 1)));
+    BDDA_HIP(hipMemcpyAsync(op->lmask, mask.data(), nxny, hipMemcpyHostToDevice, op->stream));
+    if (!ids.empty())
+        BDDA_HIP(hipMemcpyAsync(op->lines, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, op->stream));
     BDDA_HIP(hipStreamSynchronize(op->stream));
+    op->nlines = (int64_t)ids.size();
     op->c2o_host_addr.assign(address_host, address_host + nocc);
 }
 
@@ -316,7 +334,9 @@ double mbt_algorithmic_bytes(const MbtOperator* op, int s, int64_t nocc, size_t 
 {
     const MbtDims& d = op->d;
     const double f = d.f;
-    const double nxnyPz = (double)d.nx * d.ny * d.Pz;
+    // the planned f = 3 passes move only the rows of occupied (ix, iy) lines
+    const bool lined = d.f == 3 && op->lines && nocc >= 0 && has_plan(d.Py) && has_plan(d.Pz);
+    const double nxnyPz = (lined ? (double)op->nlines : (double)d.nx * d.ny) * d.Pz;
     const double nxPyPz = (double)d.nx * d.Py * d.Pz;
     const double per = s * f * c * (3.0 * nocc + 4.0 * nxnyPz + 4.0 * nxPyPz);
     return per + (double)c * op->nc * op->gh_elems;

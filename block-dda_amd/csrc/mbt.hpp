@@ -61,6 +61,12 @@ struct MbtOperator {
     int* addr_dev = nullptr;    // occupied index -> cell (the address array)
     int64_t addr_n = 0;
     std::vector<int> c2o_host_addr;  // address array the map was built from
+    // (ix, iy) lines holding at least one occupied cell: their ids
+    // (ny ix + iy, ascending) and a per-line flag; the occupied-vector passes
+    // transform only these lines along z
+    int* lines = nullptr;
+    int64_t nlines = 0;
+    unsigned char* lmask = nullptr;
     // launch timing of the last apply (ms per pass), filled when timing on
     int timing = 0;
     hipEvent_t ev[8] = {};

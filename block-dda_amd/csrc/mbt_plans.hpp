@@ -180,13 +180,19 @@ constexpr size_t zpass_lds_elems()
 // ---------------------------------------------------------------- y pass
 // in[outer][nlin][pitch] -> out[outer][nlout][pitch] over the first Zdim
 // columns; a group owns NQ consecutive columns.
+// lmask (optional): lmask[ix ny + iy] != 0 for the (ix, iy) lines that hold
+// an occupied cell, where ix = outer mod nxm and iy runs over the short side
+// (the forward pass's input, the inverse pass's output).  The rows of the
+// other lines are neither read (forward: taken as zero) nor written
+// (inverse: nothing is gathered from them).
 template <typename R, class PL, bool INV, int NT, int NQ>
 __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
                                                    int nlin, int nlout, int64_t Zdim, int64_t pitch, int64_t ntiles,
-                                                   const cplx<R>* __restrict__ twg)
+                                                   const cplx<R>* __restrict__ twg,
+                                                   const unsigned char* __restrict__ lmask, int nxm)
 {
     constexpr int P = PL::P;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* lds = tw + P;
     const int64_t outer = blockIdx.x / ntiles;
@@ -195,18 +201,32 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     const cplx<R>* src = in + (outer * nlin) * pitch + z0;
     cplx<R>* dst = out + (outer * nlout) * pitch + z0;
+    const int nshort = INV ? nlout : nlin;
+    const unsigned char* mrow = lmask ? lmask + (outer % nxm) * nshort : nullptr;
+    // the mask row in LDS (the short side is at most (P + 1) / 2 lines): the
+    // forward pass waits for it before its loads, the inverse pass reads it
+    // at its stores, after the barrier below
+    __shared__ __align__(16) unsigned char msk[((P + 1) / 2 + 15) / 16 * 16];
+    if (mrow) {
+        for (int i = threadIdx.x; i < nshort; i += NT) msk[i] = mrow[i];
+        if constexpr (!INV) __syncthreads();
+    }
     PlanFFT<R, PL, NT, NQ, 1, 1> F(tw, lds);
     const uint32_t pitch32 = (uint32_t)pitch;
     auto lane_b = [&](int j, int q) { return ((uint32_t)j * pitch32 + (uint32_t)q) * (uint32_t)sizeof(cplx<R>); };
+    // forward: the rows of unlisted lines are not read (they hold stale values)
     F.template load<INV, !INV>([&](int j, auto RB, int q, int) {
         const int p = j + decltype(RB)::value;
-        return (p < nlin && q < tz) ? *at_b(src + (int64_t)decltype(RB)::value * pitch, lane_b(j, q)) : czero<R>();
+        return (p < nlin && q < tz && (INV || !mrow || msk[p]))
+                   ? *at_b(src + (int64_t)decltype(RB)::value * pitch, lane_b(j, q))
+                   : czero<R>();
     });
     __syncthreads();
     F.template run<INV, !INV>();
     F.template store<INV, INV>([&](int j, auto RB, int q, int, cplx<R> v) {
         const int p = j + decltype(RB)::value;
-        if (p < nlout && q < tz) *at_b(dst + (int64_t)decltype(RB)::value * pitch, lane_b(j, q)) = v;
+        if (p < nlout && q < tz && (!INV || !mrow || msk[p]))
+            *at_b(dst + (int64_t)decltype(RB)::value * pitch, lane_b(j, q)) = v;
     });
 }
 
@@ -229,7 +249,7 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
 {
     constexpr int P = PL::P;
     constexpr int NC = (F == 3) ? 6 : 9;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* lds = tw + P;
     const int64_t plane = (int64_t)d.Py * Pzs;
@@ -324,13 +344,18 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
 // group's lines), which measured faster than gathering per line run.
 
 // forward + scatter: U[((rhs F + c) nxny + l) Pzs + kz] = FFT_z(X[f c2o[cell] + c])
+// lines (optional, nlines entries): the (ix, iy) lines that hold an occupied
+// cell; the groups then take TL of them each and the rows of the others are
+// not written (the y pass reads them as zero through its line mask)
 template <typename R, class PL, int F, int NT, int TL, int NCX>
 __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X, int64_t ldx,
                                                   const int* __restrict__ c2o, cplx<R>* __restrict__ U, MbtDims d,
-                                                  int Pzs, const cplx<R>* __restrict__ twg, int64_t ntiles, int nrhs)
+                                                  int Pzs, const cplx<R>* __restrict__ twg, int64_t ntiles, int nrhs,
+                                                  const int* __restrict__ lines, int64_t nlines)
 {
     constexpr int P = PL::P;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ __align__(16) int lid[(TL + 3) / 4 * 4];
     cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* lds = tw + P;
     const int64_t nxny = (int64_t)d.nx * d.ny;
@@ -338,8 +363,11 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
     int rhs;
     xcd_tile_rhs(blockIdx.x, ntiles, nrhs, tile, rhs);
     const int64_t l0 = tile * TL;
-    const int nl = (int)min((int64_t)TL, nxny - l0);
+    const int nl = (int)min((int64_t)TL, (lines ? nlines : nxny) - l0);
+    if (threadIdx.x < TL)
+        lid[threadIdx.x] = (int)threadIdx.x < nl ? (lines ? lines[l0 + threadIdx.x] : (int)(l0 + threadIdx.x)) : 0;
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
+    __syncthreads();  // line ids
     const cplx<R>* Xr = X + rhs * ldx;
     // gather of the occupied values into LDS [(line F + c) PZ + p], one
     // occupancy lookup per cell; rows of odd pitch PZ >= nz keep both the
@@ -351,7 +379,6 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
     // All occupancy lookups of a thread are issued before any of its X loads,
     // and all X loads before the LDS stores, so the dependent loads overlap
     // across the thread's cells instead of running one cell at a time.
-    const int cell0 = (int)l0 * d.nz;
     constexpr int NIT = (TL * H + NT - 1) / NT;
     int mi[NIT];
 #pragma unroll
@@ -360,7 +387,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
         const int p = e % H, tl = e / H;
         mi[it] = -1;
         if (e < TL * H && p < d.nz && tl < nl) {
-            const int cell = cell0 + tl * d.nz + p;
+            const int cell = lid[tl] * d.nz + p;
             mi[it] = c2o ? c2o[cell] : cell;
         }
     }
@@ -391,7 +418,8 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
     // through LDS, NCX components at a time, so that each wave writes 64
     // consecutive kz of one row (direct stores from the butterfly registers
     // write 16-element runs and measured slower in FP32)
-    cplx<R>* Ur = U + ((int64_t)rhs * F * nxny + l0) * Pzs;
+    // (row offsets lid Pzs are 32-bit: the launcher checks nxny Pzs elements fit)
+    cplx<R>* Ur = U + (int64_t)rhs * F * nxny * Pzs;
     static_for<0, (F + NCX - 1) / NCX>([&](auto CB) {
         constexpr int c0 = decltype(CB)::value * NCX;
         if constexpr (c0 > 0) __syncthreads();  // previous group's copy-out has read LDS
@@ -402,22 +430,25 @@ __global__ __launch_bounds__(NT) void k_zfwd_plan(const cplx<R>* __restrict__ X,
         for (int e = threadIdx.x; e < NCX * TL * P; e += NT) {
             const int p = e % P, tl = (e / P) % TL, cc = e / (P * TL);
             if (tl < nl && c0 + cc < F)
-                *at_b(Ur + (int64_t)(c0 + cc) * nxny * Pzs, ((uint32_t)tl * (uint32_t)Pzs + (uint32_t)p) *
+                *at_b(Ur + (int64_t)(c0 + cc) * nxny * Pzs, ((uint32_t)lid[tl] * (uint32_t)Pzs + (uint32_t)p) *
                                                                 (uint32_t)sizeof(cplx<R>)) = lds[e];
         }
     });
 }
 
-// inverse + gather: Y[f m + c] = diag .* X + IFFT_z(U)[cell][c]  (then * jpre)
+// lines (optional, nlines entries): only the lines that hold an occupied
+// cell are transformed (the others have nothing to gather)
 template <typename R, class PL, int F, int NT, int TL, int NCX>
 __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U, const int* __restrict__ c2o,
                                                   const cplx<R>* __restrict__ X, int64_t ldx,
                                                   const cplx<R>* __restrict__ diag, const cplx<R>* __restrict__ jpre,
                                                   cplx<R>* __restrict__ Y, int64_t ldy, MbtDims d, int Pzs,
-                                                  const cplx<R>* __restrict__ twg, int64_t ntiles, int nrhs)
+                                                  const cplx<R>* __restrict__ twg, int64_t ntiles, int nrhs,
+                                                  const int* __restrict__ lines, int64_t nlines)
 {
     constexpr int P = PL::P;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ __align__(16) int lid[(TL + 3) / 4 * 4];
     cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* lds = tw + P;
     const int64_t nxny = (int64_t)d.nx * d.ny;
@@ -425,26 +456,29 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
     int rhs;
     xcd_tile_rhs(blockIdx.x, ntiles, nrhs, tile, rhs);
     const int64_t l0 = tile * TL;
-    const int nl = (int)min((int64_t)TL, nxny - l0);
+    const int nl = (int)min((int64_t)TL, (lines ? nlines : nxny) - l0);
+    if (threadIdx.x < TL)
+        lid[threadIdx.x] = (int)threadIdx.x < nl ? (lines ? lines[l0 + threadIdx.x] : (int)(l0 + threadIdx.x)) : 0;
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
-    const cplx<R>* Ur = U + ((int64_t)rhs * F * nxny + l0) * Pzs;
+    __syncthreads();  // line ids
+    // (row offsets lid Pzs are 32-bit: the launcher checks nxny Pzs elements fit)
+    const cplx<R>* Ur = U + (int64_t)rhs * F * nxny * Pzs;
     PlanFFT<R, PL, NT, TL, F, NCX> Z(tw, lds);
     Z.template load<true>([&](int j, auto RB, int q, int c) {
         const cplx<R>* row = Ur + ((int64_t)c * nxny * Pzs + decltype(RB)::value);
-        return q < nl ? *at_b(row, ((uint32_t)q * (uint32_t)Pzs + (uint32_t)j) * (uint32_t)sizeof(cplx<R>))
+        return q < nl ? *at_b(row, ((uint32_t)lid[q] * (uint32_t)Pzs + (uint32_t)j) * (uint32_t)sizeof(cplx<R>))
                       : czero<R>();
     });
     __syncthreads();  // twiddles
     Z.template run<true>();
     const cplx<R>* Xr = X + rhs * ldx;
     cplx<R>* Yr = Y + rhs * ldy;
-    const int cell0 = (int)l0 * d.nz;
     // occupancy lookups of all the thread's cells first, then the updates
     Z.template store_keyed<true, true>(
         [&](int j, auto RB, int q) {
             const int p = j + decltype(RB)::value;
             if (p >= d.nz || q >= nl) return -1;
-            const int cell = cell0 + q * d.nz + p;
+            const int cell = lid[q] * d.nz + p;
             return c2o ? c2o[cell] : cell;
         },
         [&](int m, int c, cplx<R> v) {
@@ -456,19 +490,27 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
 }
 
 // ---------------------------------------------------------------- launchers
+// the planned y kernel serves (P, pitch): a plan exists and its lane offsets fit 32 bits
+template <typename R>
+inline bool lines_plan_ok(int P, int64_t pitch)
+{
+    return has_plan(P) && (uint64_t)P * pitch * sizeof(cplx<R>) < (1ull << 32);
+}
+
 template <typename R, bool INV>
 inline bool launch_lines_plan(const cplx<R>* in, cplx<R>* out, int nlin, int nlout, int64_t Zdim, int64_t pitch,
-                              int64_t outer, int P, const cplx<R>* tw, hipStream_t st)
+                              int64_t outer, int P, const cplx<R>* tw, hipStream_t st,
+                              const unsigned char* lmask = nullptr, int nxm = 1)
 {
     using T = PlanTiles<R>;
-    if ((uint64_t)P * pitch * sizeof(cplx<R>) >= (1ull << 32)) return false;  // 32-bit lane offsets
+    if (!lines_plan_ok<R>(P, pitch)) return false;
     return with_plan(P, [&](auto pl) {
         using PL = decltype(pl);
         const int64_t ntiles = ceil_div(Zdim, T::NQ_Y);
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_Y);
         constexpr int NT = plan_threads<PL, T::NQ_Y, 4, 256>();
         hipLaunchKernelGGL((k_lines_plan<R, PL, INV, NT, T::NQ_Y>), dim3((unsigned)(ntiles * outer)), dim3(NT),
-                           shm, st, in, out, nlin, nlout, Zdim, pitch, ntiles, tw);
+                           shm, st, in, out, nlin, nlout, Zdim, pitch, ntiles, tw, lmask, nxm);
         BDDA_HIP(hipGetLastError());
     });
 }
@@ -502,23 +544,34 @@ inline bool launch_xfused_plan(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, 
     return planned && fits;
 }
 
+// the planned z kernels serve the grid: 32-bit cell indices, lane offsets into
+// the occupied rows and row offsets into one U plane
+template <typename R, int F>
+inline bool zplan_ok(const MbtDims& d, int Pzs)
+{
+    const uint64_t cells = (uint64_t)d.nx * d.ny * d.nz;
+    return has_plan(d.Pz) && cells < (1ull << 31) && cells * F * sizeof(cplx<R>) < (1ull << 32) &&
+           (uint64_t)d.nx * d.ny * Pzs * sizeof(cplx<R>) < (1ull << 32);
+}
+
+// lines: null, or the nlines (ix, iy) lines that hold an occupied cell
 template <typename R, int F>
 inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx<R>* U, const MbtDims& d, int Pzs,
-                             const cplx<R>* tw, int s, hipStream_t st)
+                             const cplx<R>* tw, int s, hipStream_t st, const int* lines = nullptr,
+                             int64_t nlines = 0)
 {
     using T = PlanTiles<R>;
-    // 32-bit cell indices and lane offsets into the occupied rows
-    const uint64_t cells = (uint64_t)d.nx * d.ny * d.nz;
-    if (cells >= (1ull << 31) || cells * F * sizeof(cplx<R>) >= (1ull << 32)) return false;
+    if (!zplan_ok<R, F>(d, Pzs)) return false;
     return with_plan(d.Pz, [&](auto pl) {
         using PL = decltype(pl);
         constexpr int TL = T::TL_Z;  // more lines would cut the resident groups (LDS gather rows)
         const int64_t nxny = (int64_t)d.nx * d.ny;
-        const int64_t ntiles = ceil_div(nxny, TL);
+        const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
+        if (ntiles == 0) return;
         const size_t shm = sizeof(cplx<R>) * (PL::P + zpass_lds_elems<PL, F, TL, T::NCX_Z>());
         constexpr int NT = plan_threads<PL, TL, 2, 256>();
         hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
-                           X, ldx, c2o, U, d, Pzs, tw, ntiles, s);
+                           X, ldx, c2o, U, d, Pzs, tw, ntiles, s, lines, nlines);
         BDDA_HIP(hipGetLastError());
     });
 }
@@ -526,21 +579,21 @@ inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx
 template <typename R, int F>
 inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X, int64_t ldx, const cplx<R>* diag,
                              const cplx<R>* jpre, cplx<R>* Y, int64_t ldy, const MbtDims& d, int Pzs,
-                             const cplx<R>* tw, int s, hipStream_t st)
+                             const cplx<R>* tw, int s, hipStream_t st, const int* lines = nullptr,
+                             int64_t nlines = 0)
 {
     using T = PlanTiles<R>;
-    // 32-bit cell indices and lane offsets into the occupied rows
-    const uint64_t cells = (uint64_t)d.nx * d.ny * d.nz;
-    if (cells >= (1ull << 31) || cells * F * sizeof(cplx<R>) >= (1ull << 32)) return false;
+    if (!zplan_ok<R, F>(d, Pzs)) return false;
     return with_plan(d.Pz, [&](auto pl) {
         using PL = decltype(pl);
         constexpr int TL = plan_lines<PL, T::TL_Z, 2, 256>();
         const int64_t nxny = (int64_t)d.nx * d.ny;
-        const int64_t ntiles = ceil_div(nxny, TL);
+        const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
+        if (ntiles == 0) return;
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)T::NCX_Z * TL * PL::P);
         constexpr int NT = plan_threads<PL, TL, 2, 256>();
         hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
-                           U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles, s);
+                           U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles, s, lines, nlines);
         BDDA_HIP(hipGetLastError());
     });
 }

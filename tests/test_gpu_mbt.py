@@ -78,6 +78,49 @@ def test_occupied_matvec_vs_oracle(bdda, orc, f):
     assert rel(Yp, ref.reshape(addr.size, f, 3)[perm].reshape(-1, 3)) <= TOL64
 
 
+@pytest.mark.parametrize("jac", [False, True])
+def test_occupied_matvec_sparse_lines(bdda, orc, jac):
+    """Occupied matvec on the planned passes (14x16x16 -> 32^3) when most
+    (ix, iy) lines are empty: the passes then transform only the occupied
+    lines.  Occupied lines include the first and the last line, a line whose
+    only cell is its last z, and a full line; the rest are random."""
+    print("case sparse lines jacobi", jac, flush=True)
+    rng = np.random.default_rng(31)
+    n = (14, 16, 16)
+    nxny, nz = n[0] * n[1], n[2]
+    lines = set(rng.choice(nxny, size=40, replace=False).tolist()) | {0, nxny - 1, 77, 150}
+    cells = []
+    for l in sorted(lines):
+        if l == 77:
+            cells.append(l * nz + nz - 1)
+        elif l == 150:
+            cells.extend(l * nz + np.arange(nz))
+        else:
+            cells.extend(l * nz + rng.choice(nz, size=int(rng.integers(1, nz)), replace=False))
+    addr = np.array(sorted(cells), dtype=np.int32)
+    f = 3
+    alpha = rng.standard_normal(addr.size) * 0.1 + 0.3 + 0.05j
+    diag = np.repeat(1.0 / alpha, f)
+    lf, k = 0.4, 1.7
+    op = bdda.MBT_fft_init(n, f, lf, k)
+    assert op.grid == (32, 32, 32)
+    Au = orc.mbt_fft_init(n, f, lf, k)
+    op.set_target(addr, diag)
+    X = rnd(rng, f * addr.size, 3)
+    ref = orc.block_matvec(n, f, addr, Au, diag, X, jacobi=jac)
+    assert rel(op.apply_occupied(X, "fp64", jac), ref) <= TOL64
+    assert rel(op.apply_occupied(X, "fp32", jac), ref) <= TOL32
+    # the same operator after a denser target, then back: the line lists follow the target
+    addr2 = np.sort(rng.choice(int(np.prod(n)), size=900, replace=False)).astype(np.int32)
+    diag2 = np.repeat(1.0 / (rng.standard_normal(addr2.size) * 0.1 + 0.3 + 0.05j), f)
+    op.set_target(addr2, diag2)
+    X2 = rnd(rng, f * addr2.size, 2)
+    ref2 = orc.block_matvec(n, f, addr2, Au, diag2, X2, jacobi=jac)
+    assert rel(op.apply_occupied(X2, "fp64", jac), ref2) <= TOL64
+    op.set_target(addr, diag)
+    assert rel(op.apply_occupied(X, "fp64", jac), ref) <= TOL64
+
+
 def test_properties_large(bdda):
     """Linearity and complex symmetry (f=3: x^T A y = y^T A x) on 48x40x36."""
     rng = np.random.default_rng(4)
