@@ -182,9 +182,9 @@ constexpr size_t zpass_lds_elems()
 // columns; a group owns NQ consecutive columns.
 // lmask (optional): lmask[ix ny + iy] != 0 for the (ix, iy) lines that hold
 // an occupied cell, where ix = outer mod nxm and iy runs over the short side
-// (the forward pass's input, the inverse pass's output).  The rows of the
-// other lines are neither read (forward: taken as zero) nor written
-// (inverse: nothing is gathered from them).
+// (the forward pass's input, the inverse pass's output).  The forward pass
+// takes the rows of the other lines as zero (they hold stale values); the
+// inverse pass does not write them (nothing is gathered from them).
 template <typename R, class PL, bool INV, int NT, int NQ>
 __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
                                                    int nlin, int nlout, int64_t Zdim, int64_t pitch, int64_t ntiles,
@@ -203,29 +203,74 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
     cplx<R>* dst = out + (outer * nlout) * pitch + z0;
     const int nshort = INV ? nlout : nlin;
     const unsigned char* mrow = lmask ? lmask + (outer % nxm) * nshort : nullptr;
-    // the mask row in LDS (the short side is at most (P + 1) / 2 lines): the
-    // forward pass waits for it before its loads, the inverse pass reads it
-    // at its stores, after the barrier below
-    __shared__ __align__(16) unsigned char msk[((P + 1) / 2 + 15) / 16 * 16];
-    if (mrow) {
+    // the mask row in LDS (the short side is at most (P + 1) / 2 lines), read
+    // after the barrier that follows the loads
+    constexpr int MSKN = ((P + 1) / 2 + 15) / 16 * 16;
+    __shared__ __align__(16) unsigned char msk[MSKN];
+    if (mrow)
         for (int i = threadIdx.x; i < nshort; i += NT) msk[i] = mrow[i];
-        if constexpr (!INV) __syncthreads();
-    }
-    PlanFFT<R, PL, NT, NQ, 1, 1> F(tw, lds);
+    using FFT = PlanFFT<R, PL, NT, NQ, 1, 1>;
+    FFT F(tw, lds);
     const uint32_t pitch32 = (uint32_t)pitch;
     auto lane_b = [&](int j, int q) { return ((uint32_t)j * pitch32 + (uint32_t)q) * (uint32_t)sizeof(cplx<R>); };
-    // forward: the rows of unlisted lines are not read (they hold stale values)
+    // Inverse: the flags of the thread's last-stage outputs p = j + r P / RAD
+    // are read once into bit r of `keep`, so that the stores do not each wait
+    // for an LDS read.  With more than one butterfly per thread in that stage
+    // the flags are read per store instead.
+    constexpr int SK = INV ? PL::N - 1 : 0;
+    constexpr int RADK = FFT::template rad<SK, INV>(), NBK = P / RADK;
+    constexpr bool ONE = FFT::template mb<SK, INV>() == 1;
+    unsigned keep = ~0u;
+    auto read_flags = [&]() {  // (after a barrier that follows the staging)
+        if constexpr (ONE) {
+            if (mrow) {
+                keep = 0u;
+                const int j = FFT::jof(threadIdx.x);
+                static_for<0, RADK>([&](auto RR) {
+                    constexpr int r = decltype(RR)::value;
+                    const int p = j + r * NBK;
+                    // (read unconditionally, within the array, so the reads
+                    // overlap; entries at p >= nshort are not used)
+                    const unsigned f = msk[p < MSKN ? p : MSKN - 1];
+                    keep |= (p < nshort && f) ? 1u << r : 0u;
+                });
+            }
+        }
+    };
+    auto flagged = [&](int p, auto RB) {
+        if constexpr (ONE) return ((keep >> (decltype(RB)::value / NBK)) & 1u) != 0u;
+        else return !mrow || msk[p] != 0;
+    };
     F.template load<INV, !INV>([&](int j, auto RB, int q, int) {
         const int p = j + decltype(RB)::value;
-        return (p < nlin && q < tz && (INV || !mrow || msk[p]))
-                   ? *at_b(src + (int64_t)decltype(RB)::value * pitch, lane_b(j, q))
-                   : czero<R>();
+        return (p < nlin && q < tz) ? *at_b(src + (int64_t)decltype(RB)::value * pitch, lane_b(j, q))
+                                    : czero<R>();
     });
     __syncthreads();
+    if constexpr (INV) {
+        read_flags();
+    } else if (mrow) {
+        // forward: the inputs from the rows of unlisted lines become zero
+        // (reading every row and selecting measured faster than skipping the
+        // reads, which needs the flags, and so a barrier, before the loads)
+        constexpr int NB0 = P / RADK;
+        static_for<0, FFT::template mb<0, false>()>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const int j = FFT::jof(threadIdx.x + i * NT);
+            static_for<0, RADK>([&](auto RR) {
+                constexpr int r = decltype(RR)::value;
+                if constexpr (r * NB0 < (P + 1) / 2) {
+                    const int p = j + r * NB0;
+                    const unsigned f = msk[p < MSKN ? p : MSKN - 1];
+                    if (p < nlin && !f) F.v[0][i * RADK + r] = czero<R>();
+                }
+            });
+        });
+    }
     F.template run<INV, !INV>();
     F.template store<INV, INV>([&](int j, auto RB, int q, int, cplx<R> v) {
         const int p = j + decltype(RB)::value;
-        if (p < nlout && q < tz && (!INV || !mrow || msk[p]))
+        if (p < nlout && q < tz && (!INV || flagged(p, RB)))
             *at_b(dst + (int64_t)decltype(RB)::value * pitch, lane_b(j, q)) = v;
     });
 }
