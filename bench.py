@@ -38,6 +38,8 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/mbt_matvec_agg16384_hbm_traffic.md); counters cannot be read
 # from inside this process, so the figure is carried here per workload.
 MEASURED_X_TRAFFIC = {"agg16384": 17.29e9}
+# FP32 x pass VALU wave instructions per launch (SQ_INSTS_VALU, same counter runs)
+MEASURED_X_VALU = {"agg16384": 3.43e9}
 
 
 def load_pkg():
@@ -340,6 +342,12 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_X_TRAFFIC.get(args.config),
                 "kernel": "k_xfused_plan<float> at Px=%d (x FFT fwd + G^ multiply + x FFT inv)" % P[0],
                 "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms}
+    if args.config in MEASURED_X_VALU:
+        # 4 cycles per wave64 VALU instruction on each of the 1024 SIMDs
+        valu_ms = MEASURED_X_VALU[args.config] * 4 / (1024 * 2.1e9) * 1e3
+        roofline["limiter"] = ("VALU issue (packed FP32 disabled): %.3g wave instructions per launch take "
+                               "%.2f ms at 2.1 GHz, %.0f %% of the launch" % (
+                                   MEASURED_X_VALU[args.config], valu_ms, 100 * valu_ms / dom_ms))
     serving = serving_leg(pkg, args.serving_requests, args.serving_rate) if args.serving_requests > 0 else None
     cpu = None
     if not args.no_cpu:

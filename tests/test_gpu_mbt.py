@@ -121,6 +121,39 @@ def test_occupied_matvec_sparse_lines(bdda, orc, jac):
     assert rel(op.apply_occupied(X, "fp64", jac), ref) <= TOL64
 
 
+@pytest.mark.parametrize("jac", [False, True])
+def test_occupied_lines_against_cuboid(bdda, jac):
+    """Occupied matvec (only occupied lines transformed along z) against the
+    cuboid-layout matvec (every line transformed) of the same operator, on
+    100x128x120 -> 224x256x256, where the y passes hold two butterflies per
+    thread in their first and last stages.  30 % of the (ix, iy) lines are
+    occupied, each by a random subset of its cells."""
+    print("case occupied lines vs cuboid jacobi", jac, flush=True)
+    rng = np.random.default_rng(41)
+    n = (100, 128, 120)
+    f = 3
+    nxny, nz = n[0] * n[1], n[2]
+    lines = np.sort(rng.choice(nxny, size=int(0.3 * nxny), replace=False))
+    cells = [l * nz + np.flatnonzero(rng.random(nz) < 0.5) for l in lines]
+    addr = np.concatenate(cells).astype(np.int32)
+    m = addr.size
+    diag = np.repeat(1.0 / (rng.standard_normal(m) * 0.1 + 0.3 + 0.05j), f)
+    op = bdda.MBT_fft_init(n, f, 0.02, 2.5)
+    assert op.grid == (224, 256, 256)
+    op.set_target(addr, diag)
+    X = rnd(rng, f * m, 2)
+    C = int(np.prod(n))
+    xc = np.zeros((f * C, 2), dtype=complex)
+    idx = (f * addr[:, None] + np.arange(f)[None, :]).ravel()
+    xc[idx] = X
+    yc = op.apply(xc)
+    ref = diag[:, None] * X + yc[idx]
+    if jac:
+        ref = ref / diag[:, None]
+    assert rel(op.apply_occupied(X, "fp64", jac), ref) <= TOL64
+    assert rel(op.apply_occupied(X, "fp32", jac), ref) <= TOL32
+
+
 def test_properties_large(bdda):
     """Linearity and complex symmetry (f=3: x^T A y = y^T A x) on 48x40x36."""
     rng = np.random.default_rng(4)
