@@ -227,15 +227,35 @@ __global__ __launch_bounds__(NT_RED) void k_gram_mfma(const cplx<R>* __restrict_
     MfmaAcc<R> acc;
     const int64_t rb = blockIdx.x * rows_per_blk;
     const int64_t re = min(rows, rb + rows_per_blk);
-    for (int64_t r0 = rb; r0 < re; r0 += MF_TR) {
+    // Row tiles are fetched into registers one tile ahead: the next tile's
+    // loads are in flight while the current one is staged and multiplied.
+    // Tile rows beyond the block and columns beyond a (b) are zero.
+    constexpr int NL = MF_TR * 16 / NT_RED;  // elements per thread and operand
+    static_assert(MF_TR * 16 % NT_RED == 0 && NT_RED % MF_TR == 0, "tile split");
+    cplx<R> pa[NL], pb[SYM ? 1 : NL];
+    auto fetch = [&](int64_t r0) {
         const int nr = (int)min((int64_t)MF_TR, re - r0);
-        // tile rows beyond nr and columns beyond a (b) are zero
-        for (int e = tid; e < MF_TR * 16; e += NT_RED) {
-            const int r = e % MF_TR, col = e / MF_TR;
-            As[r * MF_PITCH + col] = (r < nr && col < a) ? A[col * lda + r0 + r] : czero<R>();
-            if constexpr (!SYM) Bs[r * MF_PITCH + col] = (r < nr && col < b) ? B[col * ldb + r0 + r] : czero<R>();
+        const int r = tid % MF_TR;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int col = tid / MF_TR + k * (NT_RED / MF_TR);
+            pa[k] = (r < nr && col < a) ? A[col * lda + r0 + r] : czero<R>();
+            if constexpr (!SYM) pb[k] = (r < nr && col < b) ? B[col * ldb + r0 + r] : czero<R>();
+        }
+    };
+    if (rb < re) fetch(rb);
+    for (int64_t r0 = rb; r0 < re; r0 += MF_TR) {
+        {
+            const int r = tid % MF_TR;
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const int col = tid / MF_TR + k * (NT_RED / MF_TR);
+                As[r * MF_PITCH + col] = pa[k];
+                if constexpr (!SYM) Bs[r * MF_PITCH + col] = pb[k];
+            }
         }
         __syncthreads();
+        if (r0 + MF_TR < re) fetch(r0 + MF_TR);
         const int col = lane & 15;
 #pragma unroll
         for (int t = 0; t < MF_TR / 16; ++t) {
@@ -288,14 +308,33 @@ __global__ __launch_bounds__(NT_RED) void k_gram_mfma_wide(const cplx<R>* __rest
     MfmaAcc<R> acc[TPW];
     const int64_t rb = blockIdx.x * rows_per_blk;
     const int64_t re = min(rows, rb + rows_per_blk);
-    for (int64_t r0 = rb; r0 < re; r0 += MF_TRW) {
+    // row tiles fetched into registers one tile ahead, as in k_gram_mfma
+    constexpr int NL = MF_TRW * WC / NT_RED;  // elements per thread and operand
+    static_assert(MF_TRW * WC % NT_RED == 0 && NT_RED % MF_TRW == 0, "tile split");
+    cplx<R> pa[NL], pb[SYM ? 1 : NL];
+    auto fetch = [&](int64_t r0) {
         const int nr = (int)min((int64_t)MF_TRW, re - r0);
-        for (int e = tid; e < MF_TRW * WC; e += NT_RED) {
-            const int r = e % MF_TRW, col = e / MF_TRW;
-            As[r * PITCH + col] = (r < nr && col < a) ? A[col * lda + r0 + r] : czero<R>();
-            if constexpr (!SYM) Bs[r * PITCH + col] = (r < nr && col < b) ? B[col * ldb + r0 + r] : czero<R>();
+        const int r = tid % MF_TRW;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int col = tid / MF_TRW + k * (NT_RED / MF_TRW);
+            pa[k] = (r < nr && col < a) ? A[col * lda + r0 + r] : czero<R>();
+            if constexpr (!SYM) pb[k] = (r < nr && col < b) ? B[col * ldb + r0 + r] : czero<R>();
+        }
+    };
+    if (rb < re) fetch(rb);
+    for (int64_t r0 = rb; r0 < re; r0 += MF_TRW) {
+        {
+            const int r = tid % MF_TRW;
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const int col = tid / MF_TRW + k * (NT_RED / MF_TRW);
+                As[r * PITCH + col] = pa[k];
+                if constexpr (!SYM) Bs[r * PITCH + col] = pb[k];
+            }
         }
         __syncthreads();
+        if (r0 + MF_TRW < re) fetch(r0 + MF_TRW);
         const int col = lane & 15;
 #pragma unroll
         for (int u = 0; u < TPW; ++u) {
