@@ -14,7 +14,8 @@ solve in pure FP64 is timed beside it.
 Printed JSON line: metric/value (mixed time-to-solution, seconds), the FP64
 time-to-solution and the speedup, a roofline object for the dominant kernel
 (the fused x pass of the FP32 block matvec, HIP events on the library
-stream), and a cpu_baseline measured on this host with the oracle (the FP64
+stream), the sphere workload's cross sections and S11/S12 against Mie
+theory, and a cpu_baseline measured on this host with the oracle (the FP64
 C restatement of the reference algorithm: one 1-D FFT of length
 f^2 prod(2n-1) per RHS per matvec, OpenMP over RHS columns).
 """
@@ -192,6 +193,35 @@ def observables_dev(pkg, op, dB, dX, waves, s, nang=181):
     return {"Cext": out[0], "Cabs": out[1], "Csca": out[2], "S11": S11, "S12": S12}
 
 
+def sphere_leg(pkg):
+    """The sphere workload (16 dipoles across, m = 1.33, ka = 1, x- and
+    y-polarised waves along +z) solved in mixed precision on the GPU, its cross
+    sections and S11/S12 (scattering plane xz) against Mie theory for the
+    volume-equivalent sphere."""
+    mie = importlib.import_module("block_dda_amd.mie")
+    W = importlib.import_module("block_dda_amd.workloads").build("sphere16")
+    t, k = W.target, W.k
+    op = pkg.MBT_fft_init(t.n, 3, t.lf, k)
+    op.set_target(t.address, W.diag)
+    X, info, _ = op.solve(W.B, 1e-5, 500, method=pkg.COCG_RQ, jacobi=True, mode=pkg.MODE_MIXED)
+    xs = op.cross_sections(W.B, X)
+    S11, S12, _ = op.scattering_matrix(X, [[0, 0, 1.0]], [[1.0, 0, 0]])
+    aeff = (3 * t.nocc * t.lf ** 3 / (4 * math.pi)) ** (1 / 3)
+    M = mie.mie(W.cfg["m"], k * aeff)
+    M11, M12 = mie.scattering_matrix(M["S1"], M["S2"])
+    g = math.pi * aeff ** 2
+    out = {"workload": W.describe(), "a_eff": aeff, "x_eff": k * aeff, "iters": info["iters"],
+           "true_res": info["true_res"]}
+    for q, c in (("Qext", "Cext"), ("Qabs", "Cabs"), ("Qsca", "Csca")):
+        out[q] = (xs[c] / g).tolist()
+        out[q + "_mie"] = M[q]
+        out[q + "_err_rel_to_Qext"] = float(np.abs(xs[c] / g - M[q]).max() / M["Qext"])
+    out["S11_max_err_rel_to_max"] = float(np.abs(S11[0] - M11).max() / M11.max())
+    out["S12_max_err_rel_to_S11max"] = float(np.abs(S12[0] - M12).max() / M11.max())
+    log("sphere vs Mie: %s" % out)
+    return out
+
+
 def serving_leg(pkg, nreq, rate):
     """Poisson stream of solve requests through the serving loop (mixed
     precision, block COCG-rq-Jacobi, tol 1e-5): p50/p99 latency, solves/s."""
@@ -242,6 +272,7 @@ def main():
     ap.add_argument("--itermax", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-sphere", action="store_true", help="skip the sphere-against-Mie leg")
     ap.add_argument("--matvec-reps", type=int, default=5)
     ap.add_argument("--matvec-only", action="store_true",
                     help="time the block matvecs and exit (for counter runs)")
@@ -349,6 +380,7 @@ def main():
                                "%.2f ms at 2.1 GHz, %.0f %% of the launch" % (
                                    MEASURED_X_VALU[args.config], valu_ms, 100 * valu_ms / dom_ms))
     serving = serving_leg(pkg, args.serving_requests, args.serving_rate) if args.serving_requests > 0 else None
+    sphere = None if args.no_sphere else sphere_leg(pkg)
     cpu = None
     if not args.no_cpu:
         try:
@@ -378,6 +410,7 @@ def main():
         "matvec": {kk: {"pass_ms": v["pass_ms"], "matvec_GBs": v["matvec_GBs"],
                         "pass_GBs": v["pass_GBs"]} for kk, v in mv.items()},
         "roofline": roofline,
+        "sphere_vs_mie": sphere,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
