@@ -77,3 +77,25 @@ def test_fullsize_mixed_solve_matches_fp64(bdda, big):
     S11d, S12d, _ = op.scattering_matrix(X64, u, e1)
     assert np.abs(S11m - S11d).max() <= 1e-4 * S11d.max()
     assert np.abs(S12m - S12d).max() <= 1e-4 * S11d.max()
+
+
+def test_fullsize_bicgstab_mixed_matches_fp64(bdda, big):
+    """The workload's second solver, block BiCGSTAB with Jacobi
+    (bl_bicgstab_jacobi_mvp_fft.cpp:12-97), in mixed precision against the same
+    solver in FP64: true residuals at tol, cross sections within 1e-4."""
+    W, op = big
+    B = W.B
+    tol = 1e-5
+    Xm, im, _ = op.solve(B, tol, 3000, method=bdda.BICGSTAB, jacobi=True, mode=bdda.MODE_MIXED)
+    print("bicgstab mixed", im, flush=True)
+    assert im["converged"] == 1 and im["true_res"] <= tol
+    X64, i64, _ = op.solve(B, tol, 3000, method=bdda.BICGSTAB, jacobi=True, mode=bdda.MODE_FP64)
+    print("bicgstab fp64", i64, flush=True)
+    assert i64["converged"] == 1
+    R = B - op.apply_occupied(X64)
+    assert np.linalg.norm(R) / np.linalg.norm(B) <= 1.001 * tol
+    assert rel(Xm, X64) <= 1e-3
+    xm = op.cross_sections(B, Xm)
+    x64 = op.cross_sections(B, X64)
+    for q in ("Cext", "Cabs", "Csca"):
+        assert np.abs(xm[q] - x64[q]).max() <= 1e-4 * np.abs(x64[q]).max(), q
