@@ -202,7 +202,7 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
     const int64_t outer = blockIdx.x / ntiles;
     const int64_t z0 = (blockIdx.x % ntiles) * (int64_t)NQ;
     const int tz = (int)min((int64_t)NQ, Zdim - z0);
-    for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
+    if constexpr (INV) for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     const cplx<R>* src = in + (outer * nlin) * pitch + z0;
     cplx<R>* dst = out + (outer * nlout) * pitch + z0;
     const int nshort = INV ? nlout : nlin;
@@ -211,9 +211,14 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
     constexpr bool SKIP = !INV && sizeof(R) == 8;
     constexpr int MSKN = ((P + 1) / 2 + 15) / 16 * 16;
     __shared__ __align__(16) unsigned char msk[MSKN];
-    if (mrow) {
-        for (int i = threadIdx.x; i < nshort; i += NT) msk[i] = mrow[i];
-        if constexpr (SKIP) __syncthreads();
+    auto stage_flags = [&]() {
+        if (mrow)
+            for (int i = threadIdx.x; i < nshort; i += NT) msk[i] = mrow[i];
+    };
+    if constexpr (INV) stage_flags();
+    if constexpr (SKIP) {
+        stage_flags();
+        if (mrow) __syncthreads();
     }
     using FFT = PlanFFT<R, PL, NT, NQ, 1, 1>;
     FFT F(tw, lds);
@@ -255,6 +260,14 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
                    ? *at_b(src + (int64_t)decltype(RB)::value * pitch, lane_b(j, q))
                    : czero<R>();
     });
+    if constexpr (!INV) {
+        // forward: the twiddle table (and the FP32 flag row) staged behind the
+        // row loads already issued, so that the loads do not wait for it
+        // (y forward 2.78 -> 2.72 ms in FP32; the same order measured slower
+        // in the y inverse, x and z passes)
+        for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
+        if constexpr (!SKIP) stage_flags();
+    }
     __syncthreads();
     if constexpr (INV) {
         read_flags();
