@@ -222,6 +222,72 @@ def sphere_leg(pkg):
     return out
 
 
+def sweep_leg(pkg, nwl, method, workers=4, sequential=True):
+    """The wavelength sweep on the 4096-monomer aggregate (6 dipoles across,
+    s = 8, m(lambda) linear from 2.0 + 1.0i at 300 nm to 2.4 + 0.5i at
+    1000 nm): nwl wavelengths uniform in 300-1000 nm, each with its own
+    operator (G^ at its k) and DIAG_A.  Each wavelength is solved to true
+    residual 1e-5 one after another, in mixed precision and in FP64 (device
+    buffers, operator build included), and then all wavelengths at once
+    through the serving loop, where up to four operators and their solves
+    overlap on their own streams."""
+    WL = importlib.import_module("block_dda_amd.workloads")
+    S = importlib.import_module("block_dda_amd.serving")
+    L = pkg.lib()
+    wls = np.linspace(300.0, 1000.0, nwl)
+    first = WL.sweep(float(wls[0]))
+    tgt = first.target
+    works = [first] + [WL.sweep(float(w), target=tgt) for w in wls[1:]]
+    s = first.B.shape[1]
+    rows = 3 * tgt.nocc
+    log("sweep: %d wavelengths, %s" % (nwl, first.describe()))
+    dB, dX = ctypes.c_void_p(), ctypes.c_void_p()
+    pkg._check(L.bdda_dev_alloc(rows * s * 16, ctypes.byref(dB)))
+    pkg._check(L.bdda_dev_alloc(rows * s * 16, ctypes.byref(dX)))
+    per = {"mixed": [], "fp64": []}
+    for W in (works if sequential else []):
+        Bh = np.asfortranarray(W.B)
+        for name, mode in (("mixed", pkg.MODE_MIXED), ("fp64", pkg.MODE_FP64)):
+            pkg._check(L.bdda_device_sync())
+            t0 = time.perf_counter()
+            op = pkg.MBT_fft_init(tgt.n, 3, tgt.lf, W.k)
+            op.set_target(tgt.address, W.diag)
+            pkg._check(L.bdda_memcpy_h2d(dB, Bh.ctypes.data, rows * s * 16))
+            r = solve_dev(pkg, op, dB, dX, s, 1e-5, 3000, mode, method)
+            wall = time.perf_counter() - t0
+            op.close()
+            per[name].append({"wl": W.cfg["wl"], "s": wall, "iters": r["iters"], "matvecs": r["matvecs"],
+                              "true_res": r["true_res"], "converged": r["converged"]})
+        log("sweep %.1f nm: mixed %.3f s (%d its), fp64 %.3f s (%d its)" % (
+            W.cfg["wl"], per["mixed"][-1]["s"], per["mixed"][-1]["iters"], per["fp64"][-1]["s"],
+            per["fp64"][-1]["iters"]))
+    L.bdda_dev_free(dB)
+    L.bdda_dev_free(dX)
+    out = {"wavelengths": nwl, "workload": first.describe(), "method": method}
+    for name in (per if sequential else []):
+        v = per[name]
+        it = [x["iters"] for x in v]
+        out[name] = {"total_s": float(sum(x["s"] for x in v)), "mean_s": float(np.mean([x["s"] for x in v])),
+                     "iters_min": int(min(it)), "iters_max": int(max(it)),
+                     "true_res_max": float(max(x["true_res"] for x in v)),
+                     "all_converged": bool(all(x["converged"] for x in v))}
+    if sequential:
+        out["speedup_mixed_vs_fp64"] = out["fp64"]["total_s"] / out["mixed"]["total_s"]
+    # all wavelengths at once through the serving loop (mixed precision)
+    sv = S.Server(S.GpuBackend(pkg, method=method), hbm_budget=200 << 30, workers=workers, max_cols=64).start()
+    t0 = time.perf_counter()
+    for W in works:
+        sv.submit(S.Request(S.Scene(tgt, W.k, W.diag, name="sweep%.0f" % W.cfg["wl"]), W.B, tol=1e-5))
+    sv.drain()
+    span = time.perf_counter() - t0
+    st = sv.stats()
+    sv.stop()
+    out["served"] = {"workers": workers, "makespan_s": span, "p50_s": st.get("p50_s"), "p99_s": st.get("p99_s"),
+                     "failed": st.get("failed"), "max_concurrent_batches": st.get("max_concurrent_batches")}
+    log("sweep served: %s" % out["served"])
+    return out
+
+
 def serving_leg(pkg, nreq, rate):
     """Poisson stream of solve requests through the serving loop (mixed
     precision, block COCG-rq-Jacobi, tol 1e-5): p50/p99 latency, solves/s."""
@@ -265,7 +331,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="agg16384", choices=["agg16384", "agg1024", "agg4096", "agg256", "sphere16"])
-    ap.add_argument("--method", default="cocg_rq", choices=["cocg_rq", "bicgstab", "bicggr"],
+    ap.add_argument("--method", default=None, choices=["cocg_rq", "bicgstab", "bicggr"],
                     help="block solver (Jacobi variant); the wavelength-sweep workload agg4096 is solved "
                          "with bicggr in its parity tests")
     ap.add_argument("--tol", type=float, default=1e-5)
@@ -273,6 +339,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-sphere", action="store_true", help="skip the sphere-against-Mie leg")
+    ap.add_argument("--sweep-workers", type=int, default=4, help="serving-loop workers for the sweep")
+    ap.add_argument("--sweep-served-only", action="store_true", help="skip the sweep's one-by-one solves")
+    ap.add_argument("--sweep", type=int, default=0,
+                    help="run the wavelength sweep over this many wavelengths (300-1000 nm) instead of the solve "
+                         "benchmark, with --method (default bicggr for the sweep), and print its summary")
     ap.add_argument("--matvec-reps", type=int, default=5)
     ap.add_argument("--matvec-only", action="store_true",
                     help="time the block matvecs and exit (for counter runs)")
@@ -289,7 +360,14 @@ def main():
               "that is not run here).", file=sys.stderr)
         sys.exit(2)
     pkg = load_pkg()
+    if args.method is None:
+        args.method = "bicggr" if args.sweep > 0 else "cocg_rq"
     method = {"cocg_rq": pkg.COCG_RQ, "bicgstab": pkg.BICGSTAB, "bicggr": pkg.BICGGR}[args.method]
+    if args.sweep > 0:
+        out = sweep_leg(pkg, args.sweep, method, args.sweep_workers, not args.sweep_served_only)
+        out["method"] = METHOD_NAMES[args.method]
+        print(json.dumps(out), flush=True)
+        return
     W = build_workload(args.config)
     tgt, k, diag, B = W.target, W.k, W.diag, W.B
     s = B.shape[1]
