@@ -25,6 +25,7 @@ The execution backend is pluggable: `GpuBackend` drives the HIP library
 through its C ABI; tests drive the scheduler with a CPU stand-in.
 """
 import collections
+import ctypes
 import hashlib
 import math
 import queue
@@ -120,6 +121,46 @@ class GpuBackend:
     def observables(self, op, B, X):
         xs = op.cross_sections(B, X)
         return {q: v.tolist() for q, v in xs.items()}
+
+    def solve_batch(self, op, B, tol, cols):
+        """Solve a merged block and the cross sections of each request's
+        columns (widths `cols`) with B and X kept in HBM in between: B goes up
+        once, X comes down once, and each request's cross sections run on its
+        column slice of the device blocks.  Returns X, info, [observables]."""
+        pkg = self.pkg
+        L = pkg.lib()
+        B = np.asfortranarray(B, dtype=np.complex128)
+        rows, s = B.shape
+        nb = rows * s * 16
+        dB, dX = ctypes.c_void_p(), ctypes.c_void_p()
+        dp = ctypes.POINTER(ctypes.c_double)
+        pkg._check(L.bdda_dev_alloc(nb, ctypes.byref(dB)))
+        try:
+            pkg._check(L.bdda_dev_alloc(nb, ctypes.byref(dX)))
+            try:
+                pkg._check(L.bdda_memcpy_h2d(dB, B.ctypes.data, nb))
+                info = pkg.SolveInfo()
+                rc = L.bdda_solve_dev(op.h, int(self.method), int(self.jacobi), int(self.mode), dB, s, float(tol),
+                                      int(self.itermax), dX, ctypes.byref(info), None, 0)
+                if rc not in (0, 5):  # a non-converged solve still returns X
+                    pkg._check(rc)
+                X = np.empty((rows, s), dtype=np.complex128, order="F")
+                pkg._check(L.bdda_memcpy_d2h(X.ctypes.data, dX, nb))
+                obs = []
+                c0 = 0
+                for c in cols:
+                    out = [np.zeros(c) for _ in range(3)]
+                    off = c0 * rows * 16
+                    pkg._check(L.bdda_cross_sections_dev(op.h, ctypes.c_void_p(dB.value + off),
+                                                         ctypes.c_void_p(dX.value + off), int(c), pkg.ALPHA_FIRST,
+                                                         *[o.ctypes.data_as(dp) for o in out]))
+                    obs.append({"Cext": out[0].tolist(), "Cabs": out[1].tolist(), "Csca": out[2].tolist()})
+                    c0 += c
+                return X, info.as_dict(), obs
+            finally:
+                L.bdda_dev_free(dX)
+        finally:
+            L.bdda_dev_free(dB)
 
 
 class _Entry:
@@ -341,12 +382,17 @@ class Server:
                         self._reserved += need
                         self.peak_bytes = max(self.peak_bytes, self.used_bytes())
                 B = np.concatenate([r.B for r in group], axis=1) if len(group) > 1 else group[0].B
-                X, info = self.backend.solve(entry.handle, B, min(r.tol for r in group))
+                tol = min(r.tol for r in group)
+                if hasattr(self.backend, "solve_batch"):
+                    X, info, obs = self.backend.solve_batch(entry.handle, B, tol, [r.cols for r in group])
+                else:
+                    X, info = self.backend.solve(entry.handle, B, tol)
+                    obs = None
                 c0 = 0
-                for r in group:
+                for i, r in enumerate(group):
                     r.X = X[:, c0:c0 + r.cols]
                     r.info = info
-                    r.observables = self.backend.observables(entry.handle, r.B, r.X)
+                    r.observables = obs[i] if obs is not None else self.backend.observables(entry.handle, r.B, r.X)
                     c0 += r.cols
             except Exception as e:  # reported per request; the server keeps running
                 for r in group:
