@@ -121,6 +121,31 @@ def test_occupied_matvec_sparse_lines(bdda, orc, jac):
     assert rel(op.apply_occupied(X, "fp64", jac), ref) <= TOL64
 
 
+@pytest.mark.parametrize("n", [(14, 16, 10), (14, 10, 16)])
+def test_occupied_lines_mixed_plans(bdda, orc, n):
+    """Occupied matvec where only one of the y and z axes has a planned size
+    (14x16x10 -> 32x32x20: planned y passes with line flags, all-in-LDS z
+    passes over every line; 14x10x16 -> 32x20x32: planned z passes over the
+    occupied lines, all-in-LDS y passes) against the oracle."""
+    print("case mixed plans", n, flush=True)
+    rng = np.random.default_rng(43)
+    f = 3
+    nxny, nz = n[0] * n[1], n[2]
+    lines = np.sort(rng.choice(nxny, size=nxny // 3, replace=False))
+    addr = np.concatenate([l * nz + np.flatnonzero(rng.random(nz) < 0.5) for l in lines]).astype(np.int32)
+    diag = np.repeat(1.0 / (rng.standard_normal(addr.size) * 0.1 + 0.3 + 0.05j), f)
+    lf, k = 0.4, 1.7
+    op = bdda.MBT_fft_init(n, f, lf, k)
+    assert op.grid == ((32, 32, 20) if n[2] == 10 else (32, 20, 32))
+    op.set_target(addr, diag)
+    Au = orc.mbt_fft_init(n, f, lf, k)
+    X = rnd(rng, f * addr.size, 2)
+    for jac in (False, True):
+        ref = orc.block_matvec(n, f, addr, Au, diag, X, jacobi=jac)
+        assert rel(op.apply_occupied(X, "fp64", jac), ref) <= TOL64
+        assert rel(op.apply_occupied(X, "fp32", jac), ref) <= TOL32
+
+
 @pytest.mark.parametrize("jac", [False, True])
 def test_occupied_lines_against_cuboid(bdda, jac):
     """Occupied matvec (only occupied lines transformed along z) against the
