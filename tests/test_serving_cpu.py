@@ -140,3 +140,39 @@ def test_no_merge_mode_and_stream_generator(S, T):
     assert all(b > a for a, b in zip(offs, offs[1:]))
     assert {r.cols for r in rq} <= {2, 8}
     assert abs(np.mean(np.diff([0.0] + offs)) - 0.25) < 0.15
+
+
+class BatchBackend(FakeBackend):
+    """A backend whose solve_batch returns X = 3 B and, per request, the column
+    sums of its slice of X: the worker must hand each request its own slice
+    of both."""
+
+    def solve_batch(self, h, B, tol, cols):
+        X, info = self.solve(h, B, tol)
+        X = 1.5 * X
+        obs, c0 = [], 0
+        for c in cols:
+            obs.append({"Cext": np.abs(X[:, c0:c0 + c]).sum(axis=0).tolist()})
+            c0 += c
+        return X, info, obs
+
+    def observables(self, h, B, X):  # not used when solve_batch exists
+        raise AssertionError("observables called beside solve_batch")
+
+
+def test_solve_batch_slices_per_request(S, T):
+    be = BatchBackend(solve_s=0.05)
+    sv = S.Server(be, hbm_budget=10 ** 6, workers=1).start()
+    sc = scenes(S, T, 1)[0]
+    # the first request occupies the worker; the next three merge into one batch
+    reqs = [req(S, sc, c, 100 + i) for i, c in enumerate((2, 3, 1, 4))]
+    for r in reqs:
+        sv.submit(r)
+        time.sleep(0.002)
+    assert sv.drain(timeout=10)
+    sv.stop()
+    assert any(r.batch_size > 1 for r in reqs)
+    for r in reqs:
+        assert r.error is None
+        np.testing.assert_allclose(r.X, 3.0 * r.B)
+        np.testing.assert_allclose(r.observables["Cext"], np.abs(3.0 * r.B).sum(axis=0))
