@@ -146,6 +146,30 @@ def test_occupied_lines_mixed_plans(bdda, orc, n):
         assert rel(op.apply_occupied(X, "fp32", jac), ref) <= TOL32
 
 
+@pytest.mark.parametrize("n", [(96, 6, 7), (97, 5, 6)])
+def test_x_plan_192(bdda, orc, n):
+    """Occupied matvec on grids whose x axis pads to 192 (the FP32 fused x
+    pass runs it as 8 x 8 x 3, the FP64 one as 8 x 6 x 4; 97 pads to 200 or
+    more and stays off that plan) against the oracle, with and without Jacobi
+    scaling, over 3 right-hand sides."""
+    print("case x plan 192", n, flush=True)
+    rng = np.random.default_rng(47)
+    f = 3
+    C = int(np.prod(n))
+    addr = np.sort(rng.choice(C, size=C * 2 // 3, replace=False)).astype(np.int32)
+    diag = np.repeat(1.0 / (rng.standard_normal(addr.size) * 0.1 + 0.3 + 0.05j), f)
+    lf, k = 0.45, 1.3
+    op = bdda.MBT_fft_init(n, f, lf, k)
+    assert (op.grid[0] == 192) == (n[0] == 96)
+    op.set_target(addr, diag)
+    Au = orc.mbt_fft_init(n, f, lf, k)
+    X = rnd(rng, f * addr.size, 3)
+    for jac in (False, True):
+        ref = orc.block_matvec(n, f, addr, Au, diag, X, jacobi=jac)
+        assert rel(op.apply_occupied(X, "fp64", jac), ref) <= TOL64
+        assert rel(op.apply_occupied(X, "fp32", jac), ref) <= TOL32
+
+
 @pytest.mark.parametrize("jac", [False, True])
 def test_occupied_lines_against_cuboid(bdda, jac):
     """Occupied matvec (only occupied lines transformed along z) against the
