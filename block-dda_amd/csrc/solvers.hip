@@ -442,16 +442,24 @@ int solve_fp(Problem& P, int method, int jacobi, int mode, const c64* B64, c64* 
         double rel = 1.0;
         const int max_outer = 20;
         int total_iters = 0;
-        for (int o = 0; o < max_outer; ++o) {
+        // every pass measures the residual of the current x first, so the
+        // reported true residual and convergence always describe the X
+        // returned, also after the last correction
+        for (int o = 0;; ++o) {
             const double rn = norm_fro<double>(w, R64.p, rows, rows, s);
             rel = rn / bn;
             info.true_res = rel;
             if (hist) hist->push_back(-rel);  // negative marks an outer (true) residual
             if (rel <= tol) {
                 info.converged = 1;
+                rc = BDDA_OK;
                 break;
             }
-            if (o > 0 && total_iters >= itermax) break;
+            if (o == max_outer || (o > 0 && total_iters >= itermax)) {
+                rc = BDDA_ENOCONV;
+                break;
+            }
+            if (rc == BDDA_ENOCONV) break;  // the inner solve ran out of iterations: x is its best
             // inner FP32 solve to half the outer target: the FP32 recurrence
             // residual runs below the true one by up to about 2x near 1e-5
             // (measured on the 16384-monomer aggregate: 4.8e-6 recurrence ->
@@ -468,7 +476,13 @@ int solve_fp(Problem& P, int method, int jacobi, int mode, const c64* B64, c64* 
             info.matvecs += inner.matvecs;
             info.recurrence_res = inner.recurrence_res;
             ++info.outer_iters;
-            if (rc) break;
+            if (rc != BDDA_OK && rc != BDDA_ENOCONV) {
+                // breakdown or NaN in the FP32 solve: its correction is not
+                // usable.  After a first correction x is still the best
+                // iterate, so return it with its measured residual.
+                if (o > 0) rc = BDDA_ENOCONV;
+                break;
+            }
             accumulate_32_to_64(w.stream, D32.p, rows, X64, rows, rows, s, rn);
             problem_matvec<double>(P, X64, AX.p, s, 0);  // r = B - A x
             ++info.matvecs;

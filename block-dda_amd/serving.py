@@ -100,7 +100,14 @@ class GpuBackend:
     def estimate(self, scene, cols):
         """(init_peak, resident, workspace, krylov) bytes."""
         e = self.pkg.estimate_bytes(scene.target.n, scene.f, scene.target.nocc, cols, self.mode)
-        ws = e["workspace"] if not self.ws_limit else min(e["workspace"], int(self.ws_limit))
+        ws = e["workspace"]
+        if self.ws_limit:
+            # a capped handle runs its block matvecs in column chunks, but
+            # never fewer than one column, and the FP64 residual matvec's
+            # column is the widest
+            one64 = self.pkg.estimate_bytes(scene.target.n, scene.f, scene.target.nocc, 1,
+                                            self.pkg.MODE_FP64)["workspace"]
+            ws = min(ws, max(int(self.ws_limit), one64))
         return e["init_peak"], e["resident"], ws, e["krylov"]
 
     def create(self, scene):
@@ -278,17 +285,22 @@ class Server:
             best = max(best, cur)
         return best
 
-    def _evict_for(self, need):
-        """Drop idle cached operators, least recently used first, until `need` fits."""
+    def _evict_for(self, need, keep=None):
+        """Drop idle cached operators, least recently used first, until `need`
+        fits; the operator of scene `keep` (the one being admitted, whose
+        need assumes it stays) is never dropped."""
         for key in list(self._cache.keys()):
             if self.used_bytes() + need <= self.budget:
                 return True
             e = self._cache[key]
-            if not e.busy and key not in self._inflight:
-                self.backend.release(e.handle)
-                del self._cache[key]
-                self.evictions += 1
+            if key != keep and not e.busy and key not in self._inflight:
+                self._drop(key)
         return self.used_bytes() + need <= self.budget
+
+    def _drop(self, key):
+        e = self._cache.pop(key)
+        self.backend.release(e.handle)
+        self.evictions += 1
 
     def _take_batch(self, now):
         """Oldest pending scene that is idle and fits (its requests merged), or None."""
@@ -308,18 +320,29 @@ class Server:
                     break
                 group.append(r)
                 cols += r.cols
-            init_peak, base, ws, kry = self.backend.estimate(req.scene, cols)
+            est = self.backend.estimate(req.scene, cols)
+            init_peak, base, ws, kry = est
+            fresh = max(init_peak, base + ws + kry)   # need when the operator is built for this batch
             e = self._cache.get(key)
             if e is not None:
                 need = kry + max(0, ws - e.ws)
                 self._cache.move_to_end(key)
+                if self._evict_for(need, keep=key):
+                    return key, group, need, est
+                if self._reserved == 0 and fresh <= self.budget:
+                    # only this scene's idle operator is left and the batch
+                    # does not fit beside it: drop it and build it again
+                    # within the batch's own reservation
+                    self._drop(key)
+                    if self._evict_for(fresh):
+                        return key, group, fresh, est
             else:
-                need = max(init_peak, base + ws + kry)
-            if self._evict_for(need):
-                return key, group, need, (init_peak, base, ws, kry)
-            if not self._cache and self._reserved == 0:
+                need = fresh
+                if self._evict_for(need):
+                    return key, group, need, est
+            if self._reserved == 0 and fresh > self.budget and all(k == key for k in self._cache):
                 for r in group:     # alone on an idle device and still too large
-                    r.error = "needs %d bytes, budget %d" % (need, self.budget)
+                    r.error = "needs %d bytes, budget %d" % (fresh, self.budget)
                 return key, group, 0, None
             if now - req.arrival > self.max_wait_s:
                 return None         # starving: admit nothing else until it fits
