@@ -96,6 +96,8 @@ SIGNATURES = {
     "bdda_mbt_algorithmic_bytes": (ctypes.c_double, [_vp, ctypes.c_int, ctypes.c_int]),
     "bdda_cross_sections": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "bdda_cross_sections_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
+    "bdda_cross_sections_dev_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                                       _dp]),
     "bdda_residual_columns": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int, _dp]),
     "bdda_residual_columns_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _dp]),
     "bdda_scattering_matrix": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp,

@@ -492,6 +492,24 @@ int bdda_cross_sections_dev(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX,
     })
 }
 
+int bdda_cross_sections_dev_noalloc(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX, int L, int alpha_mode,
+                                    double* Cext, double* Cabs, double* Csca)
+{
+    if (!h || !dB || !dX) return fail(BDDA_EINVAL, "null pointer");
+    BDDA_TRY({
+        double k = 0;
+        int rc = obs_check(h, L, &k);
+        if (rc) return rc;
+        if (alpha_mode != BDDA_ALPHA_FIRST && alpha_mode != BDDA_ALPHA_EACH) return fail(BDDA_EINVAL, "bad alpha_mode");
+        if (!Cext || !Cabs || !Csca) return fail(BDDA_EINVAL, "null pointer");
+        if (L > XS_MAX_COLS) return fail(BDDA_EINVAL, "bdda_cross_sections_dev_noalloc takes at most 256 columns");
+        cross_sections_noalloc(h->prob, reinterpret_cast<const c64*>(dB), reinterpret_cast<const c64*>(dX), L, k,
+                               alpha_mode == BDDA_ALPHA_EACH, xs_scratch_device(), Cext, Cabs);
+        for (int l = 0; l < L; ++l) Csca[l] = Cext[l] - Cabs[l];
+        return BDDA_OK;
+    })
+}
+
 int bdda_scattering_matrix(bdda_mbt* h, const bdda_c64* X, int64_t ldx, int npairs, const double* u,
                            const double* e1, int nang, double* S11, double* S12, bdda_c64* S)
 {

@@ -2,6 +2,9 @@
 // far-field amplitude matrix of a solved dipole block, on the device in FP64.
 #include "observables.hpp"
 #include <cmath>
+#include <mutex>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace bdda {
@@ -67,6 +70,73 @@ void cross_sections(Problem& P, const c64* dB, const c64* dX, int L, double k, i
         }
         Cext[l] = 4.0 * M_PI * k * im;
         Cabs[l] = 4.0 * M_PI * k * ab;
+    }
+}
+
+// ------------------------------------------------ cross sections, no allocation
+// Partial sums of k_xs_partial for callers that cannot allocate device
+// memory: a file-scope device array holds them (XS_MAX_COLS columns of at
+// most 256 groups, 3 sums each: 1.5 MiB), followed by the per-column totals.
+// Every GPU the library runs on holds its own copy until the process ends.
+constexpr int XS_MAX_GROUPS = 256;
+static __device__ __align__(256) double scratch[(size_t)XS_MAX_COLS * (XS_MAX_GROUPS * 3 + 2)];
+
+// out[l] and out[L + l] = sums over groups g (in order) of partial[(l G + g) 3 + 1]
+// and of partial[(l G + g) 3 + 2]: the same order as the host sum of cross_sections
+__global__ void k_xs_total(const double* __restrict__ partial, int ngroups, int L, double* __restrict__ out)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L) return;
+    double im = 0, ab = 0;
+    for (int g = 0; g < ngroups; ++g) {
+        im += partial[((int64_t)l * ngroups + g) * 3 + 1];
+        ab += partial[((int64_t)l * ngroups + g) * 3 + 2];
+    }
+    out[l] = im;
+    out[L + l] = ab;
+}
+
+double* xs_scratch_device()
+{
+    // hipGetSymbolAddress once per device, with that device current; the
+    // array's host-side name is not a device address and is never passed on
+    static std::mutex mu;
+    static std::vector<double*> ptr;
+    int dev = 0;
+    BDDA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ptr.size() <= dev) ptr.resize(dev + 1, nullptr);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        BDDA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = static_cast<double*>(p);
+    }
+    return ptr[dev];
+}
+
+void cross_sections_noalloc(Problem& P, const c64* dB, const c64* dX, int L, double k, int alpha_each,
+                            double* dscratch, double* Cext, double* Cabs)
+{
+    if (L < 1 || L > XS_MAX_COLS) throw std::invalid_argument("cross sections without allocation take 1.." +
+                                                              std::to_string(XS_MAX_COLS) + " columns");
+    const int64_t rows = P.rows();
+    const int nblk = (int)std::min<int64_t>(XS_MAX_GROUPS, std::max<int64_t>(1, ceil_div(rows, NT_OBS)));
+    double* dpart = dscratch;
+    double* dsum = dscratch + (size_t)XS_MAX_COLS * XS_MAX_GROUPS * 3;
+    hipStream_t st = P.work.stream;
+    hipLaunchKernelGGL(k_xs_partial, dim3(nblk, L), dim3(NT_OBS), 0, st, dB, dX, P.diag64, rows, P.f, alpha_each,
+                       2.0 * k * k * k / 3.0, dpart);
+    BDDA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_xs_total, dim3((unsigned)ceil_div(L, 64)), dim3(64), 0, st, dpart, nblk, L, dsum);
+    BDDA_HIP(hipGetLastError());
+    // the two planes of totals go straight into the caller's arrays (no
+    // staging buffer) and are scaled there as cross_sections scales them
+    BDDA_HIP(hipMemcpyAsync(Cext, dsum, sizeof(double) * L, hipMemcpyDeviceToHost, st));
+    BDDA_HIP(hipMemcpyAsync(Cabs, dsum + L, sizeof(double) * L, hipMemcpyDeviceToHost, st));
+    BDDA_HIP(hipStreamSynchronize(st));
+    for (int l = 0; l < L; ++l) {
+        Cext[l] = 4.0 * M_PI * k * Cext[l];
+        Cabs[l] = 4.0 * M_PI * k * Cabs[l];
     }
 }
 

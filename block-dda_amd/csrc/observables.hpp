@@ -15,6 +15,16 @@ namespace bdda {
 void cross_sections(Problem& P, const c64* dB, const c64* dX, int L, double k, int alpha_each, double* Cext,
                     double* Cabs);
 
+// cross_sections without any device allocation: the partial sums go to
+// dscratch (from xs_scratch_device(), the device address of a file-scope
+// array on the current device) instead of a per-call hipMalloc.  L is
+// 1..XS_MAX_COLS; results are bit-identical to cross_sections.  Calls on one
+// GPU share that array, so calls that overlap on one GPU race.
+constexpr int XS_MAX_COLS = 256;
+double* xs_scratch_device();
+void cross_sections_noalloc(Problem& P, const c64* dB, const c64* dX, int L, double k, int alpha_each,
+                            double* dscratch, double* Cext, double* Cabs);
+
 // Far-field amplitudes of npairs column pairs (2p: incident polarisation
 // e1_p, 2p+1: e2_p = u_p x e1_p, both along u_p) in the plane spanned by u_p
 // and e1_p at theta_a = pi a / (nang - 1): the Bohren-Huffman amplitude

@@ -198,6 +198,16 @@ int bdda_cross_sections_dev(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX,
                             double* Cabs, double* Csca);
 int bdda_scattering_matrix_dev(bdda_mbt* h, const bdda_c64* dX, int npairs, const double* u, const double* e1,
                                int nang, double* S11, double* S12, bdda_c64* S);
+/* bdda_cross_sections_dev without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The per-group partial sums
+ * live in a file-scope device array of the library (1.5 MiB per GPU, held
+ * until the process ends) instead of a per-call hipMalloc; L is limited to
+ * 1..256 columns (BDDA_EINVAL beyond).  Results are bit-identical to
+ * bdda_cross_sections_dev.  All calls on one GPU share that array, so calls
+ * that overlap on one GPU (several threads or streams) race; serialise them. */
+int bdda_cross_sections_dev_noalloc(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX, int L, int alpha_mode,
+                                    double* Cext, double* Cabs, double* Csca);
 /* per-pass kernel timing of the following applies (HIP events on the
  * handle's stream); pass_ms receives z-fwd, y-fwd, x-fused, y-inv, z-inv */
 int bdda_mbt_set_timing(bdda_mbt* h, int on);
