@@ -311,6 +311,58 @@ def serving_leg(pkg, nreq, rate):
     return st
 
 
+def reference_leg(pkg, sizes, wl0=0.3):
+    """The reference's own published workloads: the BC aggregates
+    DDA_target_0_agg_N{size} at wl0, f = 6, four seeded incident waves,
+    Mie-a_1/b_1 polarizabilities, block BiCGSTAB-Jacobi to the reference's
+    tol 1e-2 (Solver.cpp:69-73, Solver.hpp:27-28), built from the fixture
+    copies of the reference's target files and material table.  Reports the
+    GPU solve time in FP64 (the reference's arithmetic and stopping rule)
+    and in mixed precision (true FP64 residual <= 1e-2), the reference's
+    published solve time on its CPU (context only: other hardware, whole
+    seconds) and the relative errors of the FP64 result against the
+    published mean Csca, MAC and SSA."""
+    R = importlib.import_module("block_dda_amd.results")
+    M = importlib.import_module("block_dda_amd.materials")
+    gold = os.path.join(ROOT, "tests", "golden")
+    data = os.path.join(gold, "reference_data")
+    secs = json.load(open(os.path.join(gold, "reference_results.json")))["sections"]
+    mats = [M.Material.named("BC", data), M.Material.constant(2.25, 1.0, 1.0)]
+    out = []
+    for N in sizes:
+        fname = "DDA_target_0_agg_N%d_kf1.0_Df2.8.out" % N
+        s = [x for x in secs if x["file"] == "DDA_result_BC_rpp20nm.dat" and x["target_fname"] == fname
+             and abs(x["wl0"] - wl0) < 1e-12][0]
+        t0 = time.perf_counter()
+        inp = R.SectionInputs(os.path.join(data, fname), mats, 20e-3, M.Material.vacuum(), wl0, s["N_inc"], 6)
+        t_inputs = time.perf_counter() - t0
+        t = inp.target
+        t0 = time.perf_counter()
+        op = pkg.MBT_fft_init(t.n, 6, t.lf, inp.k)
+        op.set_target(t.address, inp.diag)
+        pkg._check(pkg.lib().bdda_device_sync())
+        t_init = time.perf_counter() - t0
+        row = {"target": fname, "wl0": wl0, "n": list(t.n), "nocc": t.nocc, "f": 6, "L": s["N_inc"],
+               "padded": list(op.grid), "inputs_s": t_inputs, "init_s": t_init,
+               "published_solver_time_s": s["DDA_solver_time"]}
+        for name, mode in (("fp64", pkg.MODE_FP64), ("mixed", pkg.MODE_MIXED)):
+            op.solve(inp.B, 1e-2, 300, method=pkg.BICGSTAB, jacobi=True, mode=mode)     # warm-up
+            pkg._check(pkg.lib().bdda_device_sync())
+            t0 = time.perf_counter()
+            X, info, _ = op.solve(inp.B, 1e-2, 300, method=pkg.BICGSTAB, jacobi=True, mode=mode)
+            wall = time.perf_counter() - t0
+            xs = op.cross_sections(inp.B, X, alpha_mode=pkg.ALPHA_FIRST)
+            r = inp.result(xs["Cext"], xs["Cabs"])
+            row[name] = {"solve_s": wall, "iters": info["iters"], "matvecs": info["matvecs"],
+                         "true_res": info["true_res"], "converged": info["converged"],
+                         "err_vs_published": {q: abs(r[q] - s[q]) / abs(s[q])
+                                              for q in ("mean_Csca", "mean_MAC", "mean_SSA", "MAC_mie")}}
+        op.close()
+        log("reference section: %s" % row)
+        out.append(row)
+    return out
+
+
 def compare_observables(mx, ref, geom_cs):
     """Errors of the mixed-precision observables against the FP64 solve's."""
     res = {}
@@ -350,6 +402,10 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=40,
                     help="requests in the serving stream leg (0 skips it)")
     ap.add_argument("--serving-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--reference-sizes", default="16384",
+                    help="comma-separated monomer counts of the reference's published BC aggregates to solve "
+                         "(f = 6, BiCGSTAB-Jacobi, tol 1e-2); empty skips the leg")
+    ap.add_argument("--reference-only", action="store_true", help="run only the published-workload leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes: the known-size "
                          "reference for HBM byte counters")
@@ -367,6 +423,10 @@ def main():
         out = sweep_leg(pkg, args.sweep, method, args.sweep_workers, not args.sweep_served_only)
         out["method"] = METHOD_NAMES[args.method]
         print(json.dumps(out), flush=True)
+        return
+    ref_sizes = [int(v) for v in args.reference_sizes.split(",") if v.strip()]
+    if args.reference_only:
+        print(json.dumps({"reference_published": reference_leg(pkg, ref_sizes)}), flush=True)
         return
     W = build_workload(args.config)
     tgt, k, diag, B = W.target, W.k, W.diag, W.B
@@ -465,6 +525,7 @@ def main():
                                    MEASURED_X_VALU[args.config], valu_ms, 100 * valu_ms / dom_ms))
     serving = serving_leg(pkg, args.serving_requests, args.serving_rate) if args.serving_requests > 0 else None
     sphere = None if args.no_sphere else sphere_leg(pkg)
+    refpub = reference_leg(pkg, ref_sizes) if ref_sizes else None
     cpu = None
     if not args.no_cpu:
         try:
@@ -495,6 +556,7 @@ def main():
                         "pass_GBs": v["pass_GBs"]} for kk, v in mv.items()},
         "roofline": roofline,
         "sphere_vs_mie": sphere,
+        "reference_published": refpub,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

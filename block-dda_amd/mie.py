@@ -19,8 +19,9 @@ recurrence from well above the truncation order, which is stable for
 absorbing spheres), n_stop = x + 4 x^(1/3) + 2 (Wiscombe's criterion).
 The reference carries the same quantities in gmie/gmie.cpp and
 gmie/gmie_coeff.cpp (also for magnetic spheres and absorbing media; this
-module covers mu = 1 and a non-absorbing medium, the case of every target
-here).  Angles: theta_j = pi j / (nang - 1), j = 0 .. nang - 1, as gmie
+module covers magnetic spheres (``coefficients_em``, the polarizabilities
+of the reference's targets) in a non-absorbing medium, the case of every
+target here).  Angles: theta_j = pi j / (nang - 1), j = 0 .. nang - 1, as gmie
 (gmie.cpp:112-115).
 """
 import numpy as np
@@ -30,10 +31,29 @@ def _nstop(x):
     return int(np.floor(x + 4.0 * x ** (1.0 / 3.0) + 2.0))
 
 
-def coefficients(m, x, nmax=None):
-    """a_n, b_n for n = 1 .. nmax (default n_stop)."""
+def coefficients_em(wl0, r_p, eper_p, mper_p=1.0, eper_m=1.0, mper_m=1.0, nmax=None):
+    """a_n, b_n of a sphere of radius r_p with relative permittivity eper_p
+    and permeability mper_p in a medium (eper_m, mper_m) at free-space
+    wavelength wl0 -- the arguments of gmie_coeff (gmie/gmie_coeff.cpp:8-103):
+    m_p = sqrt(eper_p mper_p), m_m = sqrt(eper_m mper_m), relative index
+    m = m_p / m_m, x = m_m (2 pi / wl0) r_p, and the magnetic sphere's
+    coefficients (Kerker et al., JOSA 73, 765 (1983)), in which 1/m of the
+    a_n numerator and denominator becomes mper_p / (mper_m m) and m of b_n
+    becomes mper_m m / mper_p.  The medium must be non-absorbing."""
+    m_m = np.sqrt(complex(eper_m) * complex(mper_m))
+    if abs(m_m.imag) > 1e-14 * abs(m_m):
+        raise ValueError("absorbing media are not supported")
+    m_p = np.sqrt(complex(eper_p) * complex(mper_p))
+    x = m_m.real * 2.0 * np.pi / float(wl0) * float(r_p)
+    return coefficients(m_p / m_m.real, x, nmax, mu_p=complex(mper_p), mu_m=complex(mper_m))
+
+
+def coefficients(m, x, nmax=None, mu_p=1.0, mu_m=1.0):
+    """a_n, b_n for n = 1 .. nmax (default n_stop); mu_p / mu_m is the
+    particle's relative permeability (1: the non-magnetic sphere)."""
     m = complex(m)
     x = float(x)
+    mu = complex(mu_p) / complex(mu_m)
     ns = _nstop(x) if nmax is None else int(nmax)
     mx = m * x
     nmx = int(max(ns, abs(mx))) + 16
@@ -54,8 +74,8 @@ def coefficients(m, x, nmax=None):
     xi = psi - 1j * chi
     n = np.arange(1, ns + 1)
     Dn = D[1:ns + 1]
-    ta = Dn / m + n / x
-    tb = m * Dn + n / x
+    ta = mu / m * Dn + n / x
+    tb = m / mu * Dn + n / x
     a = (ta * psi[1:] - psi[:-1]) / (ta * xi[1:] - xi[:-1])
     b = (tb * psi[1:] - psi[:-1]) / (tb * xi[1:] - xi[:-1])
     return a, b

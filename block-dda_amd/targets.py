@@ -27,18 +27,24 @@ Incident waves
   * ``plane_waves_sphere(ndir, seed)`` -- ndir seeded directions uniform on
     the sphere, two orthogonal polarisations each.
 """
+import os
+
 import numpy as np
 
 LDR_B1, LDR_B2, LDR_B3 = -1.8915316, 0.1648469, -1.7700004
 
 
 class Target:
-    def __init__(self, n, lf, address, name="", monomers=None):
+    def __init__(self, n, lf, address, name="", monomers=None, vol=None, material_index=None):
         self.n = tuple(int(v) for v in n)
         self.lf = float(lf)
         self.address = np.ascontiguousarray(address, dtype=np.int32)
         self.name = name
         self.monomers = monomers
+        # per element: physical volume and material (index into the list of
+        # materials the target was read with); None for generated lattices
+        self.vol = None if vol is None else np.asarray(vol, dtype=np.float64)
+        self.material_index = None if material_index is None else np.asarray(material_index, dtype=np.int32)
 
     @property
     def nocc(self):
@@ -57,6 +63,45 @@ class Target:
 
 
 # ---------------------------------------------------------------- targets
+
+def read_target_file(path, scale_factor=1.0, nmaterials=2):
+    """Target_manager::add_to_target_queue(file, material_0, material_1,
+    scale_factor) (Target_manager.cpp:114-202).  The file holds, one per
+    line: the primary-particle radius, the aggregate volume, the coating
+    volume, "nx ny nz", lf, then one "address vol material_index" line per
+    element (address 0-based, ny nz ix + nz iy + iz).  Lengths are scaled by
+    scale_factor (lf) and volumes by its cube; elements keep the file's order
+    (the rows of B and X follow it).  A material index outside
+    0 .. nmaterials-1 raises ValueError (the reference exits)."""
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    try:
+        pp_radius = float(lines[0])
+        agg_vol = float(lines[1])
+        coat_vol = float(lines[2])
+        n = tuple(int(v) for v in lines[3].split()[:3])
+        lf = float(lines[4]) * scale_factor
+    except (IndexError, ValueError) as e:
+        raise ValueError("wrong data format in %s: %s" % (path, e))
+    if len(n) != 3:
+        raise ValueError("wrong data format in %s: lattice size" % path)
+    addr, vol, mat = [], [], []
+    for ln in lines[5:]:
+        tok = ln.split()
+        if not tok:
+            continue
+        if len(tok) < 3:
+            raise ValueError("wrong data format in %s: %r" % (path, ln))
+        addr.append(int(tok[0]))
+        vol.append(float(tok[1]) * scale_factor ** 3)
+        mat.append(int(tok[2]))
+    mat = np.asarray(mat, dtype=np.int32)
+    if mat.size and (mat.min() < 0 or mat.max() >= nmaterials):
+        raise ValueError("unknown material in %s" % path)
+    t = Target(n, lf, np.asarray(addr, dtype=np.int32), name=os.path.basename(path), vol=vol, material_index=mat)
+    t.file_header = {"pp_radius": pp_radius, "aggregate_volume": agg_vol, "coating_volume": coat_vol}
+    return t
+
 
 def sphere_reference(nmax, diameter):
     """Target_manager::add_to_target_queue(SPHERE, nmax, {D, D, D}, ...)."""
@@ -199,30 +244,77 @@ def alpha_ldr(m, d, k, S=0.0):
     return acm / (1.0 + acm / d ** 3 * corr)
 
 
-def diag_from_alpha(alpha, nocc, f=3):
-    """DIAG_A = 1/alpha repeated per component (Solver.cpp:31-38)."""
+def diag_from_alpha(alpha, nocc, f=3, alpha_h=None):
+    """DIAG_A of Solver::prepare_DDA_matrix: f = 3, 1/alpha_E repeated over
+    the three components of each dipole (Solver.cpp:31-38); f = 6, 1/alpha_E
+    for rows 6 m .. 6 m + 2 and 1/alpha_H for rows 6 m + 3 .. 6 m + 5
+    (Solver.cpp:39-47).  alpha and alpha_h: one value or one per dipole."""
     a = np.broadcast_to(np.asarray(alpha, dtype=np.complex128), (nocc,))
-    return np.repeat(1.0 / a, f)
+    if f == 3:
+        return np.repeat(1.0 / a, 3)
+    if f != 6:
+        raise ValueError("unknown f value (3 or 6)")
+    if alpha_h is None:
+        raise ValueError("f = 6 needs the magnetic polarizability alpha_h")
+    h = np.broadcast_to(np.asarray(alpha_h, dtype=np.complex128), (nocc,))
+    return np.concatenate([np.repeat((1.0 / a)[:, None], 3, axis=1), np.repeat((1.0 / h)[:, None], 3, axis=1)],
+                          axis=1).reshape(6 * nocc)
+
+
+def alpha_mie(target, materials, wl0, medium, k, f=3):
+    """Electric (and for f = 6 magnetic) polarizability of every element from
+    its sphere's first Mie coefficients, as Target_manager::set_polarizability
+    (Target_manager.cpp:215-260): r_p = (3 vol / 4 pi)^(1/3),
+    alpha_E = 3 i a_1 / (2 k^3), alpha_H = 3 i b_1 / (2 k^3), with a_1, b_1 of
+    gmie_coeff(wl0, r_p, eper_p, mper_p, eper_m, mper_m) for the element's
+    material.  Returns (alpha_E, alpha_H or None), one value per element."""
+    from . import mie
+    if target.vol is None or target.material_index is None:
+        raise ValueError("target has no element volumes / materials (read it with read_target_file)")
+    k = complex(k)
+    eper_m, mper_m = medium.eper(wl0), medium.mper(wl0)
+    aE = np.empty(target.nocc, dtype=np.complex128)
+    aH = np.empty(target.nocc, dtype=np.complex128)
+    cache = {}
+    for i, (v, mi) in enumerate(zip(target.vol, target.material_index)):
+        key = (float(v), int(mi))
+        if key not in cache:
+            mat = materials[int(mi)]
+            r_p = np.cbrt(v * 3.0 / (4.0 * np.pi))
+            a, b = mie.coefficients_em(wl0, r_p, mat.eper(wl0), mat.mper(wl0), eper_m, mper_m)
+            cache[key] = (3.0j * a[0] / (2.0 * k ** 3), 3.0j * b[0] / (2.0 * k ** 3))
+        aE[i], aH[i] = cache[key]
+    return aE, (aH if f == 6 else None)
 
 
 # ------------------------------------------------------- incident waves
 
 class Waves:
-    def __init__(self, k, dirs, pols):
-        self.k = float(k)
+    def __init__(self, k, dirs, pols, hfields=None):
+        k = complex(k)
+        self.k = k.real if k.imag == 0.0 else k
         self.dirs = np.asarray(dirs, dtype=np.float64)   # (L, 3) unit propagation vectors
         self.pols = np.asarray(pols, dtype=np.complex128)  # (L, 3) unit polarisation vectors
+        # (L, 3) magnetic amplitudes: u x e times the medium's impedance
+        # factor sqrt(eper/mper) when not given (1 in vacuum)
+        self.hfields = (np.cross(self.dirs, self.pols) if hfields is None
+                        else np.asarray(hfields, dtype=np.complex128))
 
     @property
     def L(self):
         return self.dirs.shape[0]
 
-    def rhs(self, target):
-        """B[3 m + q, l] = e_l[q] exp(i k u_l . pos_m)  (Target_manager.cpp:267-275)."""
+    def rhs(self, target, f=3):
+        """f = 3: B[3 m + q, l] = e_l[q] exp(i k u_l . pos_m)  (Target_manager.cpp:267-275);
+        f = 6: rows 6 m .. 6 m + 2 hold e_l, rows 6 m + 3 .. 6 m + 5 hold h_l, times the
+        same phase (Target_manager.cpp:277-292)."""
+        if f not in (3, 6):
+            raise ValueError("unknown f value (3 or 6)")
         pos = target.positions()
-        phase = np.exp(1j * self.k * pos @ self.dirs.T)                 # (N, L)
-        B = phase[:, None, :] * self.pols.T[None, :, :]                  # (N, 3, L)
-        return np.asfortranarray(B.reshape(3 * target.nocc, self.L))
+        phase = np.exp(1j * self.k * (pos @ self.dirs.T))               # (N, L)
+        amp = self.pols.T if f == 3 else np.concatenate([self.pols.T, self.hfields.T], axis=0)
+        B = phase[:, None, :] * amp[None, :, :]                          # (N, f, L)
+        return np.asfortranarray(B.reshape(f * target.nocc, self.L))
 
 
 def plane_waves_xy_z(k):

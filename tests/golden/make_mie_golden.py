@@ -28,6 +28,14 @@ CASES = [
 ]
 NANG = 181
 NCOEF = 8
+# (name, wl0 [um], r_p [um], eper_p, mper_p)
+EM_CASES = [
+    ("bc_0.3um_rp20nm", 0.3, 0.02, 2.895 + 2.576j, 1.0 + 0.0j),
+    ("bc_1.0um_rp30nm", 1.0, 0.03, 4.146 + 2.501j, 1.0 + 0.0j),
+    ("magnetite_0.3um_rp40nm", 0.3, 0.04, 3.88492 + 4.0195j, 1.0 + 0.0j),
+    ("magnetic_x1", 0.5, 0.08, 2.0 + 0.3j, 1.6 + 0.2j),
+    ("magnetic_x3", 0.6, 0.3, 3.1 + 1.0j, 0.8 + 0.05j),
+]
 
 
 def main():
@@ -57,9 +65,26 @@ def main():
             "a": [[a[2 * j], a[2 * j + 1]] for j in range(nc)],
             "b": [[b[2 * j], b[2 * j + 1]] for j in range(nc)],
         })
+    # gmie_coeff with the arguments Target_manager::set_polarizability passes
+    # (wl0 [um], element radius r_p, eper_p, mper_p, vacuum medium): the
+    # reference's BC and magnetite primary particles, and magnetic spheres
+    # (mper_p != 1) that exercise the mper factors of a_n and b_n
+    out["cases_em"] = []
+    for name, wl0, rp, eper, mper in EM_CASES:
+        ep = (ctypes.c_double * 2)(eper.real, eper.imag)
+        mp = (ctypes.c_double * 2)(mper.real, mper.imag)
+        one = (ctypes.c_double * 2)(1.0, 0.0)
+        a = (ctypes.c_double * (2 * NCOEF))()
+        b = (ctypes.c_double * (2 * NCOEF))()
+        nc = lib.ref_gmie_coeff(wl0, rp, ep, mp, one, one, NCOEF, a, b)
+        out["cases_em"].append({
+            "name": name, "wl0": wl0, "r_p": rp, "eper_p": [eper.real, eper.imag], "mper_p": [mper.real, mper.imag],
+            "a": [[a[2 * j], a[2 * j + 1]] for j in range(nc)],
+            "b": [[b[2 * j], b[2 * j + 1]] for j in range(nc)],
+        })
     with open(os.path.join(HERE, "mie_gmie.json"), "w") as fh:
         json.dump(out, fh, indent=0)
-    print("wrote %d cases" % len(out["cases"]))
+    print("wrote %d cases, %d em cases" % (len(out["cases"]), len(out["cases_em"])))
 
 
 if __name__ == "__main__":
