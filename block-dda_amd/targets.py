@@ -16,12 +16,20 @@ Targets
     particle-cluster variant of the tunable algorithm of Filippov, Zurita &
     Rosner, J. Colloid Interface Sci. 229, 261 (2000)); every new monomer
     touches one monomer of the cluster and overlaps none.
+  * ``aggregate_cca(N, Df, kf, a0, seed)`` -- seeded cluster-cluster tunable
+    aggregation: particle-cluster subclusters joined in equal pairs at the
+    law's centre distance (``cca_merge``); ``aggregate_target`` chooses it
+    for Df <= CCA_MAX_DF, where the joins can hold the law.
+  * ``read_target_file(path, scale_factor)`` -- the reference's target files
+    (Target_manager.cpp:114-202).
   * ``voxelize(centers, a0, dipoles_across)`` -- cells of spacing
     d = 2 a0 / dipoles_across whose centres lie inside a monomer.
 Polarizabilities (Gaussian units, matching application_function's G)
   * ``alpha_cm`` -- Clausius-Mossotti, (3 d^3 / 4 pi)(m^2 - 1)/(m^2 + 2)
   * ``alpha_ldr`` -- lattice dispersion relation (Draine & Goodman, ApJ 405,
     685 (1993)) with S = sum_j (a_j e_j)^2 supplied by the caller.
+  * ``alpha_mie`` -- the reference's 3 i a_1 / (2 k^3) and 3 i b_1 / (2 k^3)
+    (Target_manager.cpp:215-260).
 Incident waves
   * ``plane_waves_xy_z`` -- x- and y-polarised waves along +z (config 1)
   * ``plane_waves_sphere(ndir, seed)`` -- ndir seeded directions uniform on
@@ -198,6 +206,168 @@ def aggregate(N, Df, kf, a0, seed, max_tries=400):
     return X - X.mean(axis=0)
 
 
+def _unit(v):
+    return v / np.linalg.norm(v)
+
+
+def _perp_basis(e, rng):
+    """Two unit vectors orthogonal to unit e and to each other, at a random angle."""
+    t = _unit(np.cross(e, rng.normal(size=3)))
+    return t, np.cross(e, t)
+
+
+def _circle_point(p, q, rp, rq, rng):
+    """A random point x with |x - p| = rp and |x - q| = rq (None if none exists)."""
+    d = np.linalg.norm(q - p)
+    if d < 1e-300:
+        return None
+    h = (rp * rp - rq * rq + d * d) / (2.0 * d)
+    rho2 = rp * rp - h * h
+    if rho2 < -1e-12 * rp * rp:
+        return None
+    e = (q - p) / d
+    t, _ = _perp_basis(e, rng)
+    return p + h * e + np.sqrt(max(rho2, 0.0)) * t
+
+
+def _rotation_taking(u, v, spin):
+    """Rotation taking unit u to unit v, then turning by `spin` about v."""
+    c = float(np.clip(u @ v, -1.0, 1.0))
+    w = np.cross(u, v)
+    s = np.linalg.norm(w)
+    if s < 1e-12:
+        if c > 0:
+            R = np.eye(3)
+        else:                    # half turn about any axis orthogonal to u
+            a = _unit(np.cross(u, [1.0, 0, 0] if abs(u[0]) < 0.9 else [0, 1.0, 0]))
+            R = 2.0 * np.outer(a, a) - np.eye(3)
+    else:
+        k = w / s
+        K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+        R = np.eye(3) + s * K + (1 - c) * (K @ K)
+    K = np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
+    S = np.eye(3) + np.sin(spin) * K + (1 - np.cos(spin)) * (K @ K)
+    return S @ R
+
+
+def _rg2(X):
+    c = X.mean(axis=0)
+    return float(((X - c) ** 2).sum(axis=1).mean()), c
+
+
+def cca_merge(A, B, Df, kf, a0, rng, max_tries=4000, widen_every=40):
+    """Join cluster B to cluster A (monomer centres, radius a0) so that the
+    joined cluster obeys N = kf (Rg/a0)^Df, with Rg the point-mass radius of
+    gyration: the centres of mass are set Gamma apart with
+        Gamma^2 = N^2/(nA nB) a0^2 (N/kf)^(2/Df) - N/nB RgA^2 - N/nA RgB^2,
+    N = nA + nB (Filippov, Zurita & Rosner, J. Colloid Interface Sci. 229,
+    261 (2000)); one monomer i of A and one monomer j of B touch, and no
+    other pair overlaps.  The contact pair is drawn among the pairs whose
+    distances dA, dB from their centres admit a contact at that Gamma; the
+    contact point, the position of B's centre on its circle and B's spin
+    about the contact are drawn at random, and rejected while any pair
+    overlaps.  When no pair admits a contact at Gamma (small clusters with
+    kf, Df that ask for an Rg the touching pair cannot have), Gamma is
+    clamped to the nearest distance that does; when `widen_every` draws in a
+    row overlap (compact laws), Gamma grows by 2 %.  Returns (joined
+    centres, Gamma wanted, Gamma used)."""
+    from scipy.spatial import cKDTree
+    nA, nB = A.shape[0], B.shape[0]
+    N = nA + nB
+    rgA2, cA = _rg2(A)
+    rgB2, cB0 = _rg2(B)
+    two_a = 2.0 * a0
+    g2 = N * N / (nA * nB) * a0 * a0 * (N / kf) ** (2.0 / Df) - N / nB * rgA2 - N / nA * rgB2
+    gam_want = float(np.sqrt(max(g2, 0.0)))
+    dA = np.linalg.norm(A - cA, axis=1)
+    dB = np.linalg.norm(B - cB0, axis=1)
+    # a pair (i, j) can touch at centre distance gam when
+    #   gam - dA_i - dB_j <= 2a  and  |dA_i - dB_j| - gam <= 2a;
+    # clamp gam into the range where some pair can: at most the two farthest
+    # monomers' reach, at least the closest |dA_i - dB_j| less 2a
+    sB = np.sort(dB)
+    pos = np.clip(np.searchsorted(sB, dA), 1, sB.size - 1) if sB.size > 1 else np.zeros(dA.size, dtype=int)
+    near = np.minimum(np.abs(dA - sB[pos]), np.abs(dA - sB[np.maximum(pos - 1, 0)]))
+    lo = max(0.0, float(near.min()) - two_a)
+    hi = float(dA.max() + dB.max() + two_a)
+    gam = min(max(gam_want, lo), hi)
+    slack = two_a * (1 + 1e-12)
+    lim = two_a * (1.0 - 1e-9)
+    treeA = cKDTree(A)
+    for attempt in range(max_tries):
+        if attempt and attempt % widen_every == 0:
+            # compact laws (Df near 3) can ask for clusters closer than any
+            # overlap-free contact allows: widen Gamma by 2 % per round
+            gam = min(gam * 1.02, hi)
+        ok_i = np.nonzero(gam - dA - dB.max() <= slack)[0]  # i reaches out far enough for some j
+        i = int(ok_i[rng.integers(ok_i.size)])
+        cand = np.nonzero((gam - dA[i] - dB <= slack) & (np.abs(dA[i] - dB) - gam <= slack))[0]
+        if cand.size == 0:
+            continue
+        j = int(cand[rng.integers(cand.size)])
+        ri, di, dj = A[i] - cA, dA[i], dB[j]
+        # D = |cB - r_i| closes both triangles (0, r_i, cB) and (r_i, x_j, cB)
+        Dlo = max(abs(gam - di), abs(dj - two_a))
+        Dhi = min(gam + di, dj + two_a)
+        if Dlo > Dhi * (1 + 1e-12):
+            continue
+        D = rng.uniform(Dlo, max(Dlo, Dhi))
+        if di < 1e-12 * a0:
+            cB = _unit(rng.normal(size=3)) * gam
+        else:
+            cB = _circle_point(np.zeros(3), ri, gam, D, rng)
+            if cB is None:
+                continue
+        if dj < 1e-12 * a0:
+            xj = cB
+        else:
+            xj = _circle_point(ri, cB, two_a, dj, rng)
+            if xj is None:
+                continue
+        if dj < 1e-12 * a0:
+            R = _rotation_taking(np.array([0, 0, 1.0]), _unit(rng.normal(size=3)), rng.uniform(0, 2 * np.pi))
+        else:
+            R = _rotation_taking(_unit(B[j] - cB0), _unit(xj - cB), rng.uniform(0, 2 * np.pi))
+        Bn = cA + cB + (B - cB0) @ R.T
+        d, _ = treeA.query(Bn, k=1, distance_upper_bound=lim)
+        if np.all(np.isinf(d)):
+            return np.concatenate([A, Bn]), gam_want, gam
+    raise RuntimeError("cca_merge: no contact without overlap after %d tries" % max_tries)
+
+
+def aggregate_cca(N, Df, kf, a0, seed, subcluster=8):
+    """Monomer centres of a seeded cluster-cluster tunable aggregate.
+
+    N monomers are dealt into 2^L subclusters of near-equal size (at most
+    `subcluster` monomers each), each grown by the particle-cluster variant
+    (`aggregate`) under the same law; the subclusters are then joined in
+    pairs, level by level (cluster-cluster aggregation of equal-sized
+    clusters, as in the tunable algorithm of Filippov et al. 2000), each
+    join by `cca_merge`.  Deterministic for a given seed.  Returns
+    (centres, merge log), the log holding per join (nA, nB, Gamma wanted,
+    Gamma used)."""
+    rng = np.random.default_rng(seed)
+    if N <= subcluster:
+        return aggregate(N, Df, kf, a0, int(rng.integers(1 << 62))), []
+    levels = int(np.ceil(np.log2(N / subcluster)))
+    m = 1 << levels
+    sizes = [N // m + (1 if q < N % m else 0) for q in range(m)]
+    clusters = [aggregate(sz, Df, kf, a0, int(rng.integers(1 << 62))) for sz in sizes]
+    log = []
+    while len(clusters) > 1:
+        nxt = []
+        for q in range(0, len(clusters), 2):
+            A, B = clusters[q], clusters[q + 1]
+            if B.shape[0] > A.shape[0]:
+                A, B = B, A
+            X, gw, gu = cca_merge(A, B, Df, kf, a0, rng)
+            log.append((A.shape[0], B.shape[0], gw, gu))
+            nxt.append(X)
+        clusters = nxt
+    X = clusters[0]
+    return X - X.mean(axis=0), log
+
+
 def voxelize(centers, a0, dipoles_across, name="aggregate"):
     """Cells (spacing d = 2 a0 / dipoles_across) whose centres lie inside a monomer."""
     d = 2.0 * a0 / dipoles_across
@@ -223,9 +393,31 @@ def voxelize(centers, a0, dipoles_across, name="aggregate"):
     return Target(tuple(n), d, np.sort(address), name=name, monomers=centers)
 
 
-def aggregate_target(N, Df, kf, a0, dipoles_across, seed):
-    c = aggregate(N, Df, kf, a0, seed)
-    t = voxelize(c, a0, dipoles_across, name="agg_N%d_Df%.1f_kf%.1f_seed%d" % (N, Df, kf, seed))
+# Cluster-cluster joins of equal clusters can meet N = kf (Rg/a0)^Df exactly
+# only while the law lets the two clusters sit far enough apart: the
+# centre distance it asks for is Gamma = 2 Rg sqrt(2^(2/Df) - 1), 2.15 Rg at
+# Df = 1.8 but 1.60 Rg at Df = 2.8, where two such clusters admit an
+# overlap-free contact only beyond about 2.6 Rg.  Measured (cca_merge's
+# log): every join exact at Df 1.8 / kf 1.3; a few widened joins up to
+# Df 2.0; Df 2.2 ends 8-35 % above the law's N at 4096 monomers; Df 2.5
+# cannot be joined.  Above CCA_MAX_DF the particle-cluster variant is used.
+CCA_MAX_DF = 2.2
+
+
+def aggregate_target(N, Df, kf, a0, dipoles_across, seed, method="auto"):
+    """Voxelized seeded aggregate: method "cca" (cluster-cluster,
+    `aggregate_cca`), "pc" (particle-cluster, `aggregate`) or "auto" (CCA
+    when Df <= CCA_MAX_DF, where its joins hold the law, else PC)."""
+    if method == "auto":
+        method = "cca" if Df <= CCA_MAX_DF else "pc"
+    if method == "cca":
+        c, _ = aggregate_cca(N, Df, kf, a0, seed)
+    elif method == "pc":
+        c = aggregate(N, Df, kf, a0, seed)
+    else:
+        raise ValueError("unknown aggregation method %r" % method)
+    t = voxelize(c, a0, dipoles_across, name="agg_%s_N%d_Df%.1f_kf%.1f_seed%d" % (method, N, Df, kf, seed))
+    t.aggregation = method
     return t
 
 

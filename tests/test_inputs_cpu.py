@@ -140,6 +140,100 @@ def test_sphere_reference_rule(mods):
     assert t.n == (16, 16, 16) and t.nocc == 2106 and abs(t.lf - 0.125) < 1e-15
 
 
+# ------------------------------------------------------------- aggregates
+
+def _connected_no_overlap(X, a0):
+    from scipy.spatial import cKDTree
+    tree = cKDTree(X)
+    d, _ = tree.query(X, k=2)
+    assert d[:, 1].min() >= 2 * a0 * (1 - 1e-9)                  # no two monomers overlap
+    pairs = tree.query_pairs(2 * a0 * (1 + 1e-9), output_type="ndarray")
+    parent = list(range(len(X)))
+
+    def root(i):
+        while parent[i] != i:
+            parent[i] = parent[parent[i]]
+            i = parent[i]
+        return i
+    for i, j in pairs:
+        parent[root(i)] = root(j)
+    assert len({root(i) for i in range(len(X))}) == 1             # one touching cluster
+
+
+def _law_n(X, kf, Df, a0):
+    c = X.mean(axis=0)
+    rg = np.sqrt(((X - c) ** 2).sum(axis=1).mean())
+    return kf * (rg / a0) ** Df
+
+
+def test_cca_law_at_every_join(mods):
+    """Cluster-cluster aggregation (Df 1.8, kf 1.3): every join sets the
+    centre distance of the law exactly, so each joined cluster and the whole
+    aggregate obey N = kf (Rg/a0)^Df; monomers touch and never overlap; the
+    same seed gives the same aggregate."""
+    T = mods["targets"]
+    N, Df, kf, a0 = 512, 1.8, 1.3, 20.0
+    X, log = T.aggregate_cca(N, Df, kf, a0, seed=11)
+    assert X.shape == (N, 3) and len(log) == 63                   # 64 subclusters of 8
+    for nA, nB, gw, gu in log:
+        assert abs(gu - gw) <= 1e-12 * gw
+    assert abs(_law_n(X, kf, Df, a0) - N) <= 1e-9 * N
+    _connected_no_overlap(X, a0)
+    X2, _ = T.aggregate_cca(N, Df, kf, a0, seed=11)
+    assert np.array_equal(X, X2)
+    X3, _ = T.aggregate_cca(N, Df, kf, a0, seed=12)
+    assert not np.allclose(X, X3)
+
+
+def test_cca_merge_sets_gamma(mods):
+    """One join: the joined cluster's Rg follows from the two clusters' Rg
+    and the centre distance (parallel-axis theorem), and equals the law."""
+    T = mods["targets"]
+    rng = np.random.default_rng(3)
+    A = T.aggregate(64, 1.8, 1.3, 1.0, 1)
+    B = T.aggregate(64, 1.8, 1.3, 1.0, 2)
+    X, gw, gu = T.cca_merge(A, B, 1.8, 1.3, 1.0, rng)
+    assert abs(gu - gw) <= 1e-12 * gw
+    cA, cB = X[:64].mean(axis=0), X[64:].mean(axis=0)
+    assert abs(np.linalg.norm(cB - cA) - gw) <= 1e-9 * gw
+    assert abs(_law_n(X, 1.3, 1.8, 1.0) - 128) <= 1e-9 * 128
+    _connected_no_overlap(X, 1.0)
+
+
+def test_cca_compact_law_widens_and_reports(mods):
+    """Df 2.2: joins that the law would interpenetrate are widened (logged),
+    the aggregate still touches everywhere and never overlaps."""
+    T = mods["targets"]
+    X, log = T.aggregate_cca(512, 2.2, 1.0, 20.0, seed=5)
+    assert any(gu > gw * (1 + 1e-9) for _, _, gw, gu in log)
+    _connected_no_overlap(X, 20.0)
+
+
+def test_pc_law_at_every_step(mods):
+    """Particle-cluster aggregation (the configs' Df 2.8, kf 1.0): from the
+    fourth monomer on every prefix obeys the law to rounding."""
+    T = mods["targets"]
+    N, Df, kf, a0 = 300, 2.8, 1.0, 20.0
+    X = T.aggregate(N, Df, kf, a0, 7)
+    for m in range(4, N + 1):
+        assert abs(_law_n(X[:m], kf, Df, a0) - m) <= 1e-9 * m, m
+    _connected_no_overlap(X, a0)
+    assert np.array_equal(X, T.aggregate(N, Df, kf, a0, 7))
+
+
+def test_aggregate_target_method(mods):
+    T = mods["targets"]
+    t1 = T.aggregate_target(64, 1.8, 1.3, 20.0, 4, seed=1)
+    t2 = T.aggregate_target(64, 2.8, 1.0, 20.0, 4, seed=1)
+    assert t1.aggregation == "cca" and t2.aggregation == "pc"
+    assert T.aggregate_target(64, 2.8, 1.0, 20.0, 4, seed=1, method="cca").aggregation == "cca"
+    with pytest.raises(ValueError):
+        T.aggregate_target(64, 2.8, 1.0, 20.0, 4, seed=1, method="dla")
+    # voxels: spacing 2 a0 / across, each occupied cell centre inside a monomer
+    d = 2 * 20.0 / 4
+    assert abs(t2.lf - d) < 1e-12 and t2.nocc > 0
+
+
 # ------------------------------------------------------------- targets, layouts
 
 def test_read_target_file(mods):
