@@ -288,23 +288,50 @@ def sweep_leg(pkg, nwl, method, workers=4, sequential=True):
     return out
 
 
-def serving_leg(pkg, nreq, rate):
+SERVING_BUDGET = 240 << 30      # of the MI355X's 288 GB HBM
+SERVING_WS_LIMIT = 32 << 30     # block matvecs beyond this workspace run in column chunks
+
+
+def serving_leg(pkg, nreq, rate, nscenes=24, cap=SERVING_BUDGET):
     """Poisson stream of solve requests through the serving loop (mixed
-    precision, block COCG-rq-Jacobi, tol 1e-5): p50/p99 latency, solves/s."""
+    precision, block COCG-rq-Jacobi, tol 1e-5): p50/p99 latency, solves/s.
+    The scenes follow the stream workload's distribution (N logUniform
+    64..8192 monomers, Df 1.8-2.8, kf 1.0-1.3, 4-8 dipoles across, lambda
+    300-1000 nm); a draw is kept when a 64-column batch of it fits the HBM
+    per-scene cap `cap` by bdda_estimate_bytes (operator build transient, or
+    operator + workspace under the chunking cap + Krylov blocks) and its
+    padded grid is within the passes' limit; the share of draws this rejects
+    is reported.  The server's own budget is the GPU's SERVING_BUDGET."""
     S = importlib.import_module("block_dda_amd.serving")
     T = importlib.import_module("block_dda_amd.targets")
+    be = S.GpuBackend(pkg, ws_limit=SERVING_WS_LIMIT)
     t0 = time.time()
-    scenes = S.synthetic_scenes(T, 8, seed=2026, nmin=64, nmax=8192, across=(4, 8), max_grid=128)
+    draw = {}
+    scenes = S.synthetic_scenes(T, nscenes, seed=2026, nmin=64, nmax=8192, across=(4, 8),
+                                fits=lambda t: (be.fresh_need(t, 3, 64) or float("inf")) <= cap,
+                                stats=draw)
     reqs = S.synthetic_requests(T, scenes, nreq, rate=rate, seed=2027, s_choices=(2, 8, 32, 64), tol=1e-5)
-    log("serving: %d scenes (nocc %s) built in %.1fs" % (len(scenes), [sc.meta["nocc"] for sc in scenes],
-                                                           time.time() - t0))
-    budget = 200 << 30
-    sv = S.Server(S.GpuBackend(pkg), hbm_budget=budget, workers=4, max_cols=64).start()
+    log("serving: %d scenes (nocc %s, lattices %s) built in %.1fs, %s" % (
+        len(scenes), [sc.meta["nocc"] for sc in scenes], [sc.meta["n"] for sc in scenes], time.time() - t0, draw))
+    done = [0]
+
+    def on_batch(key, group, secs):
+        done[0] += len(group)
+        log("served %d/%d: batch of %d request(s), %d columns, %d dipoles, %.2f s" % (
+            done[0], nreq, len(group), sum(r.cols for r in group), group[0].scene.target.nocc, secs))
+
+    sv = S.Server(be, hbm_budget=SERVING_BUDGET, workers=4, max_cols=64, on_batch=on_batch).start()
+    t1 = time.time()
     S.replay(sv, reqs)
     sv.drain()
     st = sv.stats()
     sv.stop()
+    st["stream_s"] = time.time() - t1
     st["rate_req_per_s"] = rate
+    st["scene_cap"] = ("64-column batch within %d GiB by bdda_estimate_bytes (block matvecs chunked beyond %d GiB "
+                       "of workspace) and padded grid within the passes' limit; server budget %d GiB" % (
+                           cap >> 30, SERVING_WS_LIMIT >> 30, SERVING_BUDGET >> 30))
+    st["scene_draws"] = draw
     st["scenes"] = [sc.meta for sc in scenes]
     st["s_choices"] = [2, 8, 32, 64]
     log("serving: %s" % {k: v for k, v in st.items() if k != "scenes"})
@@ -399,9 +426,12 @@ def main():
     ap.add_argument("--matvec-reps", type=int, default=5)
     ap.add_argument("--matvec-only", action="store_true",
                     help="time the block matvecs and exit (for counter runs)")
-    ap.add_argument("--serving-requests", type=int, default=40,
+    ap.add_argument("--serving-requests", type=int, default=200,
                     help="requests in the serving stream leg (0 skips it)")
     ap.add_argument("--serving-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serving-cap-gib", type=int, default=48,
+                    help="per-scene cap of the stream's scene draws (GiB a 64-column batch may need); 240 admits "
+                         "everything one GPU holds, but at 4 req/s that mix is more work than one GPU serves")
     ap.add_argument("--reference-sizes", default="16384",
                     help="comma-separated monomer counts of the reference's published BC aggregates to solve "
                          "(f = 6, BiCGSTAB-Jacobi, tol 1e-2); empty skips the leg")
@@ -523,7 +553,13 @@ def main():
         roofline["limiter"] = ("VALU issue (packed FP32 disabled): %.3g wave instructions per launch take "
                                "%.2f ms at 2.1 GHz, %.0f %% of the launch" % (
                                    MEASURED_X_VALU[args.config], valu_ms, 100 * valu_ms / dom_ms))
-    serving = serving_leg(pkg, args.serving_requests, args.serving_rate) if args.serving_requests > 0 else None
+    # the main workload's operator (with its pass workspace) and blocks are
+    # released before the other legs, which size their memory against the HBM
+    L.bdda_dev_free(dB)
+    L.bdda_dev_free(dX)
+    op.close()
+    serving = (serving_leg(pkg, args.serving_requests, args.serving_rate, cap=args.serving_cap_gib << 30)
+               if args.serving_requests > 0 else None)
     sphere = None if args.no_sphere else sphere_leg(pkg)
     refpub = reference_leg(pkg, ref_sizes) if ref_sizes else None
     cpu = None

@@ -323,6 +323,9 @@ int bdda_mbt_create(const int n[3], int f, double lf, bdda_c64 k, int prec_mask,
     if (n[0] < 1 || n[1] < 1 || n[2] < 1) return fail(BDDA_EINVAL, "grid dimensions must be >= 1");
     if (!(lf > 0) || !std::isfinite(lf)) return fail(BDDA_EINVAL, "lf must be positive");
     if (!(prec_mask & 3)) return fail(BDDA_EINVAL, "prec_mask selects no precision");
+    int P[3];
+    if (!mbt_padded_dims(n, f, P))
+        return fail(BDDA_EUNSUPPORTED, "padded grid beyond the passes' limit (P * f <= 3072 on every axis)");
     BDDA_TRY({
         std::unique_ptr<bdda_mbt> h(new bdda_mbt());
         BDDA_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -538,9 +541,11 @@ int bdda_estimate_bytes(const int n[3], int f, int64_t nocc, int L, int mode, in
     if (f != 3 && f != 6) return fail(BDDA_EINVAL, "unknown f value (3 or 6)");
     if (n[0] < 1 || n[1] < 1 || n[2] < 1 || nocc < 1 || L < 1) return fail(BDDA_EINVAL, "bad sizes");
     if (mode != BDDA_MODE_FP64 && mode != BDDA_MODE_FP32 && mode != BDDA_MODE_MIXED) return fail(BDDA_EINVAL, "bad mode");
+    int P[3];
+    if (!mbt_padded_dims(n, f, P))
+        return fail(BDDA_EUNSUPPORTED, "padded grid beyond the passes' limit (P * f <= 3072 on every axis)");
     BDDA_TRY({
-        MbtDims d{n[0], n[1], n[2], f, choose_padded(n[0], 8, f == 3 ? 1 : 8), choose_padded(n[1], 1, 1),
-                  choose_padded(n[2], 1, 1)};
+        MbtDims d{n[0], n[1], n[2], f, P[0], P[1], P[2]};
         const int Pzs = (int)ceil_div(d.Pz, 16) * 16;
         const int64_t nc = (f == 3) ? 6 : 9;
         const int64_t G = (int64_t)d.Px * d.Py * d.Pz;

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <stdexcept>
 
 namespace bdda {
 
@@ -37,6 +38,18 @@ int choose_padded(int n, int mult, int plan_mult)
     while (planned <= 1024 && !(has_plan(planned) && planned % plan_mult == 0)) ++planned;
     if (planned <= 1024 && planned <= smooth + smooth / 8) return planned;
     return smooth;
+}
+
+bool mbt_padded_dims(const int n[3], int f, int P[3])
+{
+    // the all-in-LDS fused x kernel (f = 6, or sizes without a plan) needs
+    // Px % 8 == 0; planned sizes serve f = 3 only
+    P[0] = choose_padded(n[0], 8, f == 3 ? 1 : 8);
+    P[1] = choose_padded(n[1], 1, 1);
+    P[2] = choose_padded(n[2], 1, 1);
+    for (int a = 0; a < 3; ++a)
+        if ((int64_t)P[a] * f > MBT_MAX_PF) return false;
+    return true;
 }
 
 AxisPlan make_axis_plan(int P, bool last8)
@@ -181,11 +194,14 @@ MbtOperator* mbt_create(const int n[3], int f, double lf, double kre, double kim
     op->d.ny = n[1];
     op->d.nz = n[2];
     op->d.f = f;
-    // the all-in-LDS fused x kernel (f = 6, or sizes without a plan) needs
-    // Px % 8 == 0; planned sizes serve f = 3 only
-    op->d.Px = choose_padded(n[0], 8, f == 3 ? 1 : 8);
-    op->d.Py = choose_padded(n[1], 1, 1);
-    op->d.Pz = choose_padded(n[2], 1, 1);
+    int P[3];
+    if (!mbt_padded_dims(n, f, P)) {
+        delete op;
+        throw std::invalid_argument("padded grid beyond the passes' limit (P * f <= 3072 on every axis)");
+    }
+    op->d.Px = P[0];
+    op->d.Py = P[1];
+    op->d.Pz = P[2];
     op->lf = lf;
     op->kre = kre;
     op->kim = kim;

@@ -35,6 +35,13 @@ struct MbtDims {
 int choose_padded(int n, int mult, int plan_mult);
 AxisPlan make_axis_plan(int P, bool last8);
 
+// The all-in-LDS passes (sizes without a compile-time plan, and f = 6) hold
+// one padded line of every component in a work-group: P f <= MBT_MAX_PF.
+constexpr int MBT_MAX_PF = 3072;
+// padded sizes of the operator for lattice n; false when an axis exceeds
+// what the passes can transform
+bool mbt_padded_dims(const int n[3], int f, int P[3]);
+
 enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1 };
 
 struct MbtOperator {

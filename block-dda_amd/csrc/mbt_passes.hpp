@@ -30,6 +30,8 @@ struct Budget<double> {
     static constexpr int MAXE = 12;
     static constexpr int SEG = 8;
 };
+static_assert(Budget<float>::MAXE * NT_PASS == MBT_MAX_PF && Budget<double>::MAXE * NT_PASS == MBT_MAX_PF,
+              "mbt_padded_dims must refuse what the all-in-LDS passes cannot hold");
 
 
 

@@ -99,16 +99,27 @@ class GpuBackend:
 
     def estimate(self, scene, cols):
         """(init_peak, resident, workspace, krylov) bytes."""
-        e = self.pkg.estimate_bytes(scene.target.n, scene.f, scene.target.nocc, cols, self.mode)
+        return self.estimate_target(scene.target.n, scene.f, scene.target.nocc, cols)
+
+    def estimate_target(self, n, f, nocc, cols):
+        e = self.pkg.estimate_bytes(n, f, nocc, cols, self.mode)
         ws = e["workspace"]
         if self.ws_limit:
             # a capped handle runs its block matvecs in column chunks, but
             # never fewer than one column, and the FP64 residual matvec's
             # column is the widest
-            one64 = self.pkg.estimate_bytes(scene.target.n, scene.f, scene.target.nocc, 1,
-                                            self.pkg.MODE_FP64)["workspace"]
+            one64 = self.pkg.estimate_bytes(n, f, nocc, 1, self.pkg.MODE_FP64)["workspace"]
             ws = min(ws, max(int(self.ws_limit), one64))
         return e["init_peak"], e["resident"], ws, e["krylov"]
+
+    def fresh_need(self, target, f, cols):
+        """Bytes a batch of `cols` columns needs when it builds the operator
+        (None when the lattice is beyond what the passes can transform)."""
+        try:
+            init_peak, base, ws, kry = self.estimate_target(target.n, f, target.nocc, cols)
+        except self.pkg.BddaError:
+            return None
+        return max(init_peak, base + ws + kry)
 
     def create(self, scene):
         t = scene.target
@@ -191,8 +202,9 @@ class Server:
     waited longer than max_wait_s blocks admissions behind it until it fits.
     """
 
-    def __init__(self, backend, hbm_budget, workers=4, max_cols=64, merge=True, max_wait_s=2.0):
+    def __init__(self, backend, hbm_budget, workers=4, max_cols=64, merge=True, max_wait_s=2.0, on_batch=None):
         self.backend = backend
+        self.on_batch = on_batch      # called by the worker with (scene key, requests, seconds) after each batch
         self.budget = int(hbm_budget)
         self.max_cols = int(max_cols)
         self.merge = merge
@@ -273,8 +285,21 @@ class Server:
             "batches": len(self.batches),
             "merged_requests": sum(len(b[1]) for b in self.batches) - len(self.batches),
             "max_concurrent_batches": self._max_overlap(),
+            "admitted_while_busy": self._admitted_while_busy(ok),
+            "refused": sum(1 for r in self.completed if r.error is not None and "budget" in r.error),
             "peak_bytes": self.peak_bytes, "budget_bytes": self.budget, "evictions": self.evictions,
         }
+
+    def _admitted_while_busy(self, reqs):
+        """Requests that arrived while another batch was on the device and
+        started before that batch finished (continuous batching)."""
+        n = 0
+        for r in reqs:
+            for key, ids, t0, t1, _ in self.batches:
+                if r.id not in ids and t0 <= r.arrival and r.start < t1:
+                    n += 1
+                    break
+        return n
 
     # ---------------------------------------------------------------- internals
     def _max_overlap(self):
@@ -320,7 +345,12 @@ class Server:
                     break
                 group.append(r)
                 cols += r.cols
-            est = self.backend.estimate(req.scene, cols)
+            try:
+                est = self.backend.estimate(req.scene, cols)
+            except Exception as e:   # e.g. a lattice beyond what the passes transform
+                for r in group:
+                    r.error = "refused: %r" % (e,)
+                return key, group, 0, None
             init_peak, base, ws, kry = est
             fresh = max(init_peak, base + ws + kry)   # need when the operator is built for this batch
             e = self._cache.get(key)
@@ -436,6 +466,8 @@ class Server:
                 self._cv.notify_all()
             for r in group:
                 r.done.set()
+            if self.on_batch is not None:
+                self.on_batch(key, group, t1 - t0)
 
 
 # -------------------------------------------------------------------- streams
@@ -444,16 +476,23 @@ def log_uniform_int(rng, lo, hi):
     return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
 
 
-def synthetic_scenes(pkg_targets, nscenes, seed, nmin=64, nmax=8192, across=(4, 8), f=3, max_grid=256):
+def synthetic_scenes(pkg_targets, nscenes, seed, nmin=64, nmax=8192, across=(4, 8), f=3, max_grid=None, fits=None,
+                     stats=None):
     """Seeded scenes: aggregates with N ~ logUniform[nmin, nmax] monomers,
     Df U[1.8, 2.8], kf U[1.0, 1.3], a0 = 20 nm, dipoles across U{across},
     wavelength U[300, 1000] nm, m = U[1.3, 2.5] + i U[0, 1]; LDR polarizability.
-    A draw whose bounding lattice exceeds max_grid cells on an axis is drawn
-    again (an open Df = 1.8 aggregate of thousands of monomers at 8 dipoles
-    across spans over a thousand cells per axis, beyond one GPU)."""
+    Aggregation: cluster-cluster where its joins hold the law (Df <= 2.2),
+    particle-cluster above (targets.aggregate_target).
+    A draw is drawn again when `fits(target)` is false (a memory cap such as
+    "a 64-column batch fits one GPU", see GpuBackend.fresh_need: an open
+    Df = 1.8 aggregate of thousands of monomers at 8 dipoles across spans
+    over a thousand cells per axis, beyond one GPU) or, if given, when its
+    lattice exceeds max_grid cells on an axis.  `stats` (a dict) receives
+    the number of draws and of rejected draws."""
     T = pkg_targets
     rng = np.random.default_rng(seed)
     scenes = []
+    draws = rejected = 0
     for i in range(nscenes):
         while True:
             N = log_uniform_int(rng, nmin, nmax)
@@ -461,16 +500,21 @@ def synthetic_scenes(pkg_targets, nscenes, seed, nmin=64, nmax=8192, across=(4, 
             kf = rng.uniform(1.0, 1.3)
             ac = int(rng.integers(across[0], across[1] + 1))
             tgt = T.aggregate_target(N, Df, kf, 20.0, ac, int(rng.integers(1 << 30)))
-            if max(tgt.n) <= max_grid:
+            draws += 1
+            if (max_grid is None or max(tgt.n) <= max_grid) and (fits is None or fits(tgt)):
                 break
+            rejected += 1
         wl = rng.uniform(300.0, 1000.0)
         m = complex(rng.uniform(1.3, 2.5), rng.uniform(0.0, 1.0))
         k = 2 * math.pi / wl
         alpha = T.alpha_ldr(m, tgt.lf, k)
         diag = T.diag_from_alpha(alpha, tgt.nocc, f)
         sc = Scene(tgt, k, diag, f, name="scene%d_N%d_wl%.0f" % (i, N, wl))
-        sc.meta = {"N": N, "Df": Df, "kf": kf, "across": ac, "wl": wl, "m": [m.real, m.imag], "nocc": tgt.nocc}
+        sc.meta = {"N": N, "Df": Df, "kf": kf, "across": ac, "wl": wl, "m": [m.real, m.imag], "nocc": tgt.nocc,
+                   "n": list(tgt.n), "aggregation": getattr(tgt, "aggregation", None)}
         scenes.append(sc)
+    if stats is not None:
+        stats.update({"draws": draws, "rejected": rejected, "rejected_share": rejected / max(draws, 1)})
     return scenes
 
 

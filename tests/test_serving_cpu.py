@@ -184,6 +184,29 @@ def test_too_large_request_fails_alone(S, T):
     assert small.error is None and np.array_equal(small.X, 2.0 * small.B)
 
 
+class RefusingBackend(FakeBackend):
+    """Estimates fail for scenes of more than 12 dipoles (as the library
+    refuses lattices beyond the passes' limit)."""
+
+    def estimate(self, scene, cols):
+        if scene.target.nocc > 12:
+            raise RuntimeError("EUNSUPPORTED: padded grid beyond the passes' limit")
+        return super().estimate(scene, cols)
+
+
+def test_unsupported_scene_is_refused(S, T):
+    be = RefusingBackend()
+    sc = scenes(S, T, 4)                  # nocc 10 .. 13
+    sv = S.Server(be, hbm_budget=10 ** 6, workers=2).start()
+    bad = sv.submit(req(S, sc[3], 2, 0))
+    good = sv.submit(req(S, sc[0], 2, 1))
+    assert sv.drain(timeout=10)
+    st = sv.stats()
+    sv.stop()
+    assert bad.error is not None and "EUNSUPPORTED" in bad.error
+    assert good.error is None and st["failed"] == 1
+
+
 def test_no_merge_mode_and_stream_generator(S, T):
     be = FakeBackend(solve_s=0.002)
     sc = scenes(S, T, 2)
