@@ -34,13 +34,44 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "time-to-solution & p99 latency at rel.res 1e-5; mixed-prec vs FP64 on MI355X"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the FP32 fused x pass, from FETCH_SIZE and
-# WRITE_SIZE counter runs scaled by a 2 GiB device copy
-# (profiles/mbt_matvec_agg16384_hbm_traffic.md); counters cannot be read
-# from inside this process, so the figure is carried here per workload.
-MEASURED_X_TRAFFIC = {"agg16384": 17.29e9}
-# FP32 x pass VALU wave instructions per launch (SQ_INSTS_VALU, same counter runs)
-MEASURED_X_VALU = {"agg16384": 3.43e9}
+# Hardware counters cannot be read from inside this process: the FP32 x
+# pass's HBM bytes per launch (FETCH_SIZE + WRITE_SIZE scaled by a 2 GiB
+# device copy) and VALU wave instructions (SQ_INSTS_VALU) come from the
+# committed profile of the workload, and only when that profile was taken on
+# the kernel sources built now (their SHA-256 is recorded in it).
+KERNEL_SOURCES = ["bdda_common.hpp", "fft_engine.hpp", "fft_plan.hpp", "mbt_plans.hpp"]
+
+
+def kernel_source_stamp():
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "block-dda_amd", "csrc", name), "rb") as fh:
+            h.update(name.encode() + b"\0" + fh.read())
+    return h.hexdigest()
+
+
+def profile_counters(config):
+    """Counter figures of profiles/mbt_matvec_<config>_hbm_traffic.md (its
+    "bench counters" block) and whether they describe the current kernel."""
+    path = os.path.join(ROOT, "profiles", "mbt_matvec_%s_hbm_traffic.md" % config)
+    if not os.path.exists(path):
+        return None, "no committed counter profile for %s" % config
+    vals, inside = {}, False
+    for line in open(path):
+        s = line.strip()
+        if s == "bench counters:":
+            inside = True
+        elif inside and ":" in s:
+            k, v = s.split(":", 1)
+            vals[k.strip()] = v.strip()
+        elif inside and not s:
+            break
+    if "source_sha256" not in vals:
+        return None, "profile has no source stamp"
+    if vals["source_sha256"] != kernel_source_stamp():
+        return None, "profile stale: taken on other x-pass sources than the ones built now"
+    return vals, "profile of the current x-pass sources"
 
 
 def load_pkg():
@@ -145,33 +176,48 @@ def residual_col_max(pkg, op, dB, dX, s):
     return float(res.max())
 
 
-def cpu_baseline(tgt, k, diag, s, n_matvecs, threads):
-    """Oracle (FP64 C restatement of the reference) on a bounded sample."""
+def host_memory():
+    """(total, available) host bytes from /proc/meminfo."""
+    vals = {}
+    with open("/proc/meminfo") as fh:
+        for line in fh:
+            k, v = line.split(":", 1)
+            vals[k] = int(v.split()[0]) * 1024
+    return vals.get("MemTotal", 0), vals.get("MemAvailable", 0)
+
+
+def cpu_baseline(tgt, s, n_matvecs, threads):
+    """The oracle's MBT matvec (FP64 C restatement of MBT_fft_mvp: BT_pad, one
+    1-D FFT of length F = 9 prod(2n - 1), the product, the inverse FFT, 1/F,
+    BT_reconstruct) on ONE column at the workload's own F, timed on this
+    host; scaled over the s columns and the FP64 GPU solve's matvec count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    T = importlib.import_module("block_dda_amd.targets")
-    # sample: the same generator and lattice spacing at 512 monomers, one
-    # block matvec of `threads` RHS columns (OpenMP over columns, like the
-    # reference); scaled to the full target by F log F of the 1-D FFT.
-    cfg = dict(N=512, Df=2.8, kf=1.0, a0=20.0, across=4, seed=512)
-    st = T.aggregate_target(cfg["N"], cfg["Df"], cfg["kf"], cfg["a0"], cfg["across"], cfg["seed"])
-    sdiag = T.diag_from_alpha(1.0 / diag[0], st.nocc, 3)
-    Au = orc.mbt_fft_init(st.n, 3, st.lf, k)
-    rng = np.random.default_rng(1)
-    X = rng.standard_normal((3 * st.nocc, threads)) + 1j * rng.standard_normal((3 * st.nocc, threads))
+    F = orc.fft_length(tgt.n, 3)
+    total, avail = host_memory()
+    # Au_til, the two F-length work vectors and the four-step transpose buffer
+    need = 4 * F * 16
+    if need > 0.8 * avail:
+        return {"value": None, "unit": "s", "cores": threads, "kind": "port",
+                "sample": "not run: one real-shape column needs %.1f GB, %.1f GB available" % (need / 1e9,
+                                                                                          avail / 1e9)}
+    orc.fft_config(1 << 22, threads)
+    # the timed product does not depend on Au_til's values, and the
+    # reference builds Au_til before its solve timer (Solver.cpp:55,59)
+    Au = np.full(F, 1e-3 + 0.0j)
+    x = np.random.default_rng(1).standard_normal(3 * int(np.prod(tgt.n))) + 0j
     t0 = time.perf_counter()
-    orc.block_matvec(st.n, 3, st.address, Au, sdiag, X, jacobi=True, nthreads=threads)
-    t_block = time.perf_counter() - t0
-    Fs = orc.fft_length(st.n, 3)
-    Ff = orc.fft_length(tgt.n, 3)
-    scale = (Ff * math.log2(Ff)) / (Fs * math.log2(Fs))
-    t_full = t_block * scale * math.ceil(s / threads) * n_matvecs
+    orc.mbt_fft_mvp(tgt.n, 3, Au, x)
+    t_col = time.perf_counter() - t0
+    del Au
     return {
-        "value": t_full, "unit": "s", "cores": threads, "kind": "port",
-        "sample": ("oracle block matvec (FP64, 1-D FFT of length F=%d, %d RHS on %d OpenMP threads) on the "
-                   "512-monomer aggregate at the same lattice spacing: %.2f s; scaled by F log F to F=%d and by "
-                   "%d RHS x %d block matvecs of the FP64 GPU solve (extrapolated, solve only)"
-                   % (Fs, threads, threads, t_block, Ff, s, n_matvecs)),
+        "value": t_col * s * n_matvecs, "unit": "s", "cores": threads, "kind": "port",
+        "sample": ("oracle MBT matvec of one column at the workload's own shape (lattice %s, 1-D FFT of length "
+                   "F=%d, four-step on %d OpenMP threads): %.1f s measured; x %d columns x %d block matvecs of "
+                   "the FP64 GPU solve (extrapolated over columns and matvecs only; solve only)"
+                   % (list(tgt.n), F, threads, t_col, s, n_matvecs)),
+        "column_s": t_col, "F": F, "host_nproc": os.cpu_count(), "host_ram_bytes": total,
+        "host_ram_available_bytes": avail,
     }
 
 
@@ -409,6 +455,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=3, help="timed FP64 solves (at least 3, median reported)")
     ap.add_argument("--config", default="agg16384", choices=["agg16384", "agg1024", "agg4096", "agg256", "sphere16"])
     ap.add_argument("--method", default=None, choices=["cocg_rq", "bicgstab", "bicggr"],
                     help="block solver (Jacobi variant); the wavelength-sweep workload agg4096 is solved "
@@ -513,9 +560,18 @@ def main():
     fp64 = None
     obs64 = None
     if not args.no_fp64:
-        fp64 = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_FP64, method)
+        # the same warm-up as the mixed solve, then at least three timed
+        # solves; the comparison uses the medians of both
+        for _ in range(args.warmup):
+            log("fp64 warmup: %s" % solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_FP64, method))
+        runs64 = []
+        for _ in range(max(3, args.fp64_steps)):
+            runs64.append(solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_FP64, method))
+            log("fp64 step: %s" % runs64[-1])
+        fp64 = dict(runs64[-1])
+        fp64["wall_s_runs"] = [r["wall_s"] for r in runs64]
+        fp64["wall_s_median"] = float(np.median(fp64["wall_s_runs"]))
         fp64["true_res_col_max"] = residual_col_max(pkg, op, dB, dX, s)
-        log("fp64 solve: %s" % fp64)
         obs64 = observables_dev(pkg, op, dB, dX, W.waves, s)
     # mixed precision (true FP64 residual <= tol): warmup + timed steps
     for _ in range(args.warmup):
@@ -538,21 +594,25 @@ def main():
         log("observables (mixed vs fp64): %s" % obs)
     ms_per_step = total / max(args.steps, 1) * 1e3
     tts = ms_per_step / 1e3
+    mixed_median = float(np.median([r["wall_s"] for r in runs]))
 
     xf = mv["fp32"]
     dom_bytes = xf["bytes"]["x_fused"]
     dom_ms = xf["pass_ms"]["x_fused"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    counters, counters_note = profile_counters(args.config)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_X_TRAFFIC.get(args.config),
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": float(counters["x_traffic_bytes"]) if counters else None,
+                "traffic_source": counters_note,
                 "kernel": "k_xfused_plan<float> at Px=%d (x FFT fwd + G^ multiply + x FFT inv)" % P[0],
                 "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms}
-    if args.config in MEASURED_X_VALU:
+    if counters:
         # 4 cycles per wave64 VALU instruction on each of the 1024 SIMDs
-        valu_ms = MEASURED_X_VALU[args.config] * 4 / (1024 * 2.1e9) * 1e3
+        valu = float(counters["x_valu_wave_insts"])
+        valu_ms = valu * 4 / (1024 * 2.1e9) * 1e3
         roofline["limiter"] = ("VALU issue (packed FP32 disabled): %.3g wave instructions per launch take "
-                               "%.2f ms at 2.1 GHz, %.0f %% of the launch" % (
-                                   MEASURED_X_VALU[args.config], valu_ms, 100 * valu_ms / dom_ms))
+                               "%.2f ms at 2.1 GHz, %.0f %% of the launch" % (valu, valu_ms, 100 * valu_ms / dom_ms))
     # the main workload's operator (with its pass workspace) and blocks are
     # released before the other legs, which size their memory against the HBM
     L.bdda_dev_free(dB)
@@ -566,7 +626,9 @@ def main():
     if not args.no_cpu:
         try:
             nmv = fp64["matvecs"] if fp64 else runs[-1]["matvecs"]
-            cpu = cpu_baseline(tgt, k, diag, s, nmv, threads=min(s, 16, os.cpu_count() or 1))
+            # the GPU box's CPU share is 16 cores
+            cpu = cpu_baseline(tgt, s, nmv, threads=min(16, os.cpu_count() or 1))
+            log("cpu baseline: %s" % cpu)
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed: %r" % e)
     out = {
@@ -579,12 +641,14 @@ def main():
         "time_to_solution_with_init_s": tts + init_s,
         "mixed": {"iters": runs[-1]["iters"], "outer": runs[-1]["outer_iters"], "matvecs": runs[-1]["matvecs"],
                   "true_res": runs[-1]["true_res"], "true_res_col_max": runs[-1]["true_res_col_max"],
-                  "converged": runs[-1]["converged"]},
-        "fp64_time_s": fp64["wall_s"] if fp64 else None,
+                  "converged": runs[-1]["converged"], "wall_s_median": mixed_median,
+                  "wall_s_runs": [r["wall_s"] for r in runs]},
+        "fp64_time_s": fp64["wall_s_median"] if fp64 else None,
         "fp64": {k2: fp64[k2] for k2 in ("iters", "matvecs", "true_res", "true_res_col_max", "recurrence_res",
-                                          "converged")}
+                                          "converged", "wall_s_median", "wall_s_runs")}
         if fp64 else None,
-        "speedup_mixed_vs_fp64": (fp64["wall_s"] / tts) if fp64 else None,
+        # medians of both, each after the same warm-up
+        "speedup_mixed_vs_fp64": (fp64["wall_s_median"] / mixed_median) if fp64 else None,
         "observables": obs,
         "serving": serving,
         "init_s": init_s,

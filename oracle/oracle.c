@@ -8,7 +8,8 @@
  *
  * Deviations that do not change results beyond rounding:
  *   - FFTW's 1-D transforms are replaced by orc_fft (own mixed radix with
- *     direct prime butterflies, Bluestein for prime factors > 61).
+ *     direct prime butterflies, Bluestein for prime factors > 61; four-step
+ *     with OpenMP threads for long lengths, see orc_fft_config).
  *   - BT_pad / BT_reconstruct / BT_fft keep the reference recursions but
  *     write into preallocated output at an offset instead of
  *     conservativeResize-and-copy.
@@ -219,6 +220,7 @@ typedef struct {
     int64_t fac[MAXFAC], rem[MAXFAC];
     zc* t0;         /* e^{-2 pi i j/n}, j < 32768 */
     zc* t1;         /* e^{-2 pi i (j*32768)/n} */
+    zc* wp[MAXFAC]; /* e^{-2 pi i e/p}, e < p, for the direct prime factor p = fac[i] */
     int nblue;
     blue_t blue[MAXFAC];
 } fftplan;
@@ -311,6 +313,12 @@ static fftplan* plan_create(int64_t n)
     for (int64_t j = 0; j < nt1; ++j) pl->t1[j] = expi_frac(j * 32768, n);
     for (int i = 0; i < pl->nfac; ++i) {
         const int64_t p = pl->fac[i];
+        if (p == 2 || p == 4 || p >= BLUE_MIN) continue;
+        pl->wp[i] = (zc*)malloc(sizeof(zc) * p);
+        for (int64_t e = 0; e < p; ++e) pl->wp[i][e] = expi_frac(e, p);
+    }
+    for (int i = 0; i < pl->nfac; ++i) {
+        const int64_t p = pl->fac[i];
         if (p < BLUE_MIN) continue;
         int found = 0;
         for (int j = 0; j < pl->nblue; ++j) if (pl->blue[j].p == p) found = 1;
@@ -324,6 +332,7 @@ static void plan_destroy(fftplan* pl)
     if (!pl) return;
     free(pl->t0);
     free(pl->t1);
+    for (int i = 0; i < pl->nfac; ++i) free(pl->wp[i]);
     for (int j = 0; j < pl->nblue; ++j) {
         free(pl->blue[j].chirp); free(pl->blue[j].w2);
         free(pl->blue[j].hhat[0]); free(pl->blue[j].hhat[1]);
@@ -332,7 +341,7 @@ static void plan_destroy(fftplan* pl)
 }
 
 /* length-p DFT of t (in place), w_p^e = plan_w((e mod p) * n/p) */
-static void dft_small(const fftplan* pl, zc* t, zc* tmp, int64_t p, int sign)
+static void dft_small(const fftplan* pl, const zc* wp, zc* t, zc* tmp, int64_t p, int sign)
 {
     if (p == 2) {
         const zc a = t[0], b = t[1];
@@ -362,17 +371,39 @@ static void dft_small(const fftplan* pl, zc* t, zc* tmp, int64_t p, int sign)
         }
         return;
     }
-    const int64_t step = pl->n / p;
-    for (int64_t q2 = 0; q2 < p; ++q2) {
-        zc acc = 0;
-        for (int64_t q = 0; q < p; ++q) {
-            zc w = plan_w(pl, ((q * q2) % p) * step);
-            if (sign > 0) w = conj(w);
-            acc += t[q] * w;
-        }
-        tmp[q2] = acc;
+    /* odd prime p: pair x_q with x_{p-q}.  With s_q = x_q + x_{p-q},
+     * d_q = x_q - x_{p-q} (q = 1 .. h, h = (p-1)/2) and W = e^{-2 pi i/p},
+     *   y_k = x_0 + sum_q s_q cos(2 pi q k/p) -+ i sum_q d_q sin(2 pi q k/p),
+     *   y_{p-k} likewise with the sign of the sine sum flipped,
+     * real coefficients only (wp holds cos - i sin). */
+    const int64_t h = (p - 1) / 2;
+    const zc x0 = t[0];
+    zc* sq = tmp;
+    zc* dq = tmp + h;
+    zc y0 = x0;
+    for (int64_t q = 1; q <= h; ++q) {
+        sq[q - 1] = t[q] + t[p - q];
+        dq[q - 1] = t[q] - t[p - q];
+        y0 += sq[q - 1];
     }
-    for (int64_t q2 = 0; q2 < p; ++q2) t[q2] = tmp[q2];
+    for (int64_t k = 1; k <= h; ++k) {
+        double ar = creal(x0), ai = cimag(x0), br = 0, bi = 0;
+        int64_t e = 0;          /* q k mod p */
+        for (int64_t q = 1; q <= h; ++q) {
+            e += k;
+            if (e >= p) e -= p;
+            const double c = creal(wp[e]), sn = -cimag(wp[e]);
+            ar += creal(sq[q - 1]) * c;
+            ai += cimag(sq[q - 1]) * c;
+            br += creal(dq[q - 1]) * sn;
+            bi += cimag(dq[q - 1]) * sn;
+        }
+        /* forward: y_k = a - i b; inverse: y_k = a + i b (b = sum d sin) */
+        const double sg = sign > 0 ? 1.0 : -1.0;
+        t[k] = (ar - sg * bi) + I * (ai + sg * br);
+        t[p - k] = (ar + sg * bi) + I * (ai - sg * br);
+    }
+    t[0] = y0;
 }
 
 static void fft_rec(const fftplan* pl, zc* out, const zc* in, int64_t fstride,
@@ -394,7 +425,7 @@ static void fft_rec(const fftplan* pl, zc* out, const zc* in, int64_t fstride,
             }
             t[q] = v;
         }
-        dft_small(pl, t, tmp, p, sign);
+        dft_small(pl, pl->wp[lev], t, tmp, p, sign);
         for (int64_t q = 0; q < p; ++q) out[u + q * m] = t[q];
     }
 }
@@ -407,9 +438,101 @@ static int64_t plan_scratch(const fftplan* pl)
     return s;
 }
 
+/* Four-step FFT (D. H. Bailey, "FFTs in external or hierarchical memory",
+ * J. Supercomputing 4, 23-35 (1990)) for long transforms: with n = n1 n2,
+ * j = j1 n2 + j2 and k = k1 + n1 k2,
+ *   X[k1 + n1 k2] = sum_j2 W_n2^(j2 k2) W_n^(j2 k1) sum_j1 W_n1^(j1 k1) x[j1 n2 + j2].
+ * Columns j2 are gathered BLK at a time (contiguous runs of x), transformed
+ * with the length-n1 plan, twiddled and stored row-major; rows k1 are then
+ * transformed with the length-n2 plan and scattered BLK at a time.  The
+ * sub-transforms stay in cache, and both loops run on g_fft_threads OpenMP
+ * threads when called outside a parallel region (one thread inside one, as
+ * in the block matvec's loop over columns). */
+static int64_t g_fourstep_min = (int64_t)1 << 24;
+static int g_fft_threads = 1;
+
+void orc_fft_config(int64_t fourstep_min, int threads)
+{
+    g_fourstep_min = fourstep_min > 0 ? fourstep_min : ((int64_t)1 << 62);
+    g_fft_threads = threads > 0 ? threads : 1;
+}
+
+/* n1, n2 > 1 with n1 n2 = n from the plan's factors, as close to sqrt(n) as
+ * a greedy split gets (0 when n is prime) */
+static int fourstep_split(const fftplan* pl, int64_t* n1, int64_t* n2)
+{
+    int64_t f[MAXFAC];
+    int nf = pl->nfac;
+    for (int i = 0; i < nf; ++i) f[i] = pl->fac[i];
+    for (int i = 0; i < nf; ++i)         /* descending */
+        for (int j = i + 1; j < nf; ++j)
+            if (f[j] > f[i]) { int64_t t = f[i]; f[i] = f[j]; f[j] = t; }
+    int64_t a = 1, b = 1;
+    for (int i = 0; i < nf; ++i) {
+        if (a <= b) a *= f[i];
+        else b *= f[i];
+    }
+    if (a < 2 || b < 2) return 0;
+    *n1 = a;
+    *n2 = b;
+    return 1;
+}
+
+static void fft_fourstep(const fftplan* pl, int64_t n1, int64_t n2, const zc* in, zc* out, int sign)
+{
+    enum { BLK = 8 };
+    const int64_t n = pl->n;
+    fftplan* p1 = plan_create(n1);
+    fftplan* p2 = plan_create(n2);
+    zc* T = (zc*)malloc(sizeof(zc) * n);
+    const int64_t nmax = n1 > n2 ? n1 : n2;
+    int64_t sc = plan_scratch(p1);
+    if (plan_scratch(p2) > sc) sc = plan_scratch(p2);
+    const int nt = omp_in_parallel() ? 1 : g_fft_threads;
+#pragma omp parallel num_threads(nt)
+    {
+        zc* col = (zc*)malloc(sizeof(zc) * BLK * nmax);
+        zc* res = (zc*)malloc(sizeof(zc) * BLK * nmax);
+        zc* t = (zc*)malloc(sizeof(zc) * sc);
+        zc* tmp = (zc*)malloc(sizeof(zc) * sc);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t jb = 0; jb < n2; jb += BLK) {
+            const int nb = (int)(n2 - jb < BLK ? n2 - jb : BLK);
+            for (int64_t j1 = 0; j1 < n1; ++j1)
+                for (int b = 0; b < nb; ++b) col[b * n1 + j1] = in[j1 * n2 + jb + b];
+            for (int b = 0; b < nb; ++b) fft_rec(p1, res + b * n1, col + b * n1, 1, 0, sign, t, tmp);
+            for (int64_t k1 = 0; k1 < n1; ++k1)
+                for (int b = 0; b < nb; ++b) {
+                    zc w = plan_w(pl, (jb + b) * k1);     /* (jb + b) k1 < n2 n1 = n */
+                    if (sign > 0) w = conj(w);
+                    T[k1 * n2 + jb + b] = res[b * n1 + k1] * w;
+                }
+        }
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t kb = 0; kb < n1; kb += BLK) {
+            const int nb = (int)(n1 - kb < BLK ? n1 - kb : BLK);
+            for (int b = 0; b < nb; ++b) fft_rec(p2, res + b * n2, T + (kb + b) * n2, 1, 0, sign, t, tmp);
+            for (int64_t k2 = 0; k2 < n2; ++k2)
+                for (int b = 0; b < nb; ++b) out[kb + b + n1 * k2] = res[b * n2 + k2];
+        }
+        free(col);
+        free(res);
+        free(t);
+        free(tmp);
+    }
+    free(T);
+    plan_destroy(p1);
+    plan_destroy(p2);
+}
+
 /* out-of-place execute: out must not alias in */
 static void plan_exec(const fftplan* pl, const zc* in, zc* out, int sign)
 {
+    int64_t n1, n2;
+    if (pl->n >= g_fourstep_min && fourstep_split(pl, &n1, &n2)) {
+        fft_fourstep(pl, n1, n2, in, out, sign);
+        return;
+    }
     const int64_t s = plan_scratch(pl);
     zc* t = (zc*)malloc(sizeof(zc) * s);
     zc* tmp = (zc*)malloc(sizeof(zc) * s);
@@ -461,11 +584,15 @@ static int mbt_mvp_with_plan(const fftplan* pl, const int* n, int f, const zc* A
     zc* a = (zc*)malloc(sizeof(zc) * F);
     zc* b = (zc*)malloc(sizeof(zc) * F);
     if (!a || !b) { free(a); free(b); return -2; }
+    const int nt = omp_in_parallel() ? 1 : g_fft_threads;
     const int64_t lp = bt_pad_rec(n, 3, f, p_hat, a);  /* (:21) */
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int64_t i = lp; i < F; ++i) a[i] = 0;         /* (:23-25) */
     plan_exec(pl, a, b, -1);                            /* (:27) */
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int64_t i = 0; i < F; ++i) b[i] *= Au_til[i];  /* (:28) */
     plan_exec(pl, b, a, +1);                            /* (:29) */
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int64_t i = 0; i < F; ++i) a[i] /= (double)F;  /* (:30) */
     bt_rec_rec(n, 3, f, a, q_hat);                      /* (:31) */
     free(a);

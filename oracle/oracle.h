@@ -49,6 +49,9 @@ int64_t orc_bt_reconstruct(const int* n, int f, const double* bz, double* b);
 
 /* in-place unnormalised DFT of length len; sign = -1 forward, +1 backward */
 int orc_fft(int64_t len, double* data, int sign);
+/* transforms of at least fourstep_min points run as four-step FFTs on
+ * `threads` OpenMP threads (outside parallel regions); defaults 2^24, 1 */
+void orc_fft_config(int64_t fourstep_min, int threads);
 
 int orc_mbt_fft_init(const int* n, int f, double lf, double kre, double kim,
                      double* Au_til);

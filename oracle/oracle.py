@@ -46,6 +46,8 @@ def lib():
         L.orc_bt_pad.restype = ctypes.c_int64
         L.orc_bt_reconstruct.restype = ctypes.c_int64
         L.orc_fft.argtypes = [ctypes.c_int64, _dp, ctypes.c_int]
+        L.orc_fft_config.argtypes = [ctypes.c_int64, ctypes.c_int]
+        L.orc_fft_config.restype = None
         L.orc_qr_reduced.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp]
         L.orc_bt_fft_custom.argtypes = [_ip, ctypes.c_int, ELEM_FN, ctypes.c_void_p, _dp]
         L.orc_block_matvec.argtypes = [_ip, ctypes.c_int, _ip, ctypes.c_int64, _dp, _dp,
@@ -125,6 +127,12 @@ def bt_reconstruct(n, f, bz):
     ln = lib().orc_bt_reconstruct(_i(_n(n)), f, _d(bz), _d(out))
     assert ln == out.size
     return out
+
+
+def fft_config(fourstep_min=1 << 24, threads=1):
+    """Transforms of at least fourstep_min points run as four-step FFTs on
+    `threads` OpenMP threads (when not inside the block matvec's own loop)."""
+    lib().orc_fft_config(int(fourstep_min), int(threads))
 
 
 def fft(x, sign=-1):

@@ -37,6 +37,38 @@ def test_fft_against_numpy(orc, L):
     assert np.abs(orc.fft(x, +1) - refi).max() <= 1e-13 * np.abs(refi).max()
 
 
+@pytest.mark.parametrize("L", [96, 3072, 3729, 30030, 38533, 127, 1 << 16])
+def test_fourstep_fft_against_numpy(orc, L):
+    """The four-step path (forced at small lengths here) against numpy,
+    with one thread and with four; primes fall back to the direct plan."""
+    rng = np.random.default_rng(L)
+    x = rng.standard_normal(L) + 1j * rng.standard_normal(L)
+    ref, refi = np.fft.fft(x), np.fft.ifft(x) * L
+    try:
+        for threads in (1, 4):
+            orc.fft_config(64, threads)
+            assert np.abs(orc.fft(x, -1) - ref).max() <= 1e-13 * np.abs(ref).max()
+            assert np.abs(orc.fft(x, +1) - refi).max() <= 1e-13 * np.abs(refi).max()
+    finally:
+        orc.fft_config()
+
+
+def test_fourstep_matvec_against_dense(orc):
+    """The MBT matvec through four-step transforms equals the dense product."""
+    n, f, lf, k = (4, 3, 5), 3, 0.7, 1.1
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(f * 60) + 1j * rng.standard_normal(f * 60)
+    A = orc.dense_offdiag(n, f, lf, k)
+    Au = orc.mbt_fft_init(n, f, lf, k)
+    try:
+        orc.fft_config(64, 3)
+        y = orc.mbt_fft_mvp(n, f, Au, x)
+    finally:
+        orc.fft_config()
+    ref = A @ x
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_bt_fft_label_kat(orc):
     kat = json.load(open(os.path.join(GOLD, "kat_label_map.json")))
     table = {(tuple(a), tuple(b)): lab for a, b, lab in kat["labels"]}
@@ -170,7 +202,11 @@ def test_solvers_direct(orc, method, f, jacobi):
     n, addr, diag, Au, B, A = _problem(orc, f)
     X, it, hist = orc.solve(method, n, f, addr, Au, diag, B, 1e-10, 400, jacobi=jacobi)
     Xd = np.linalg.solve(A, B)
-    assert hist[-1] < 1e-10 and it < 400
+    # block COCG terminates after ceil(n/L) steps in exact arithmetic; the
+    # FP64 loss of bilinear orthogonality costs about 20 % more here.  The
+    # BiCG products take up to about twice that on these non-normal systems.
+    nl = -(-A.shape[0] // B.shape[1])
+    assert hist[-1] < 1e-10 and it <= (1.3 if method == "cocg_rq" else 2.2) * nl, (it, nl)
     assert np.linalg.norm(A @ X - B) / np.linalg.norm(B) < 1e-9
     assert np.linalg.norm(X - Xd) / np.linalg.norm(Xd) < 1e-8
 
@@ -211,6 +247,59 @@ def _bicggr(A, b, tol, itmax):
     return x, hist
 
 
+def _krylov_basis(Aop, Bop, k):
+    """Orthonormal basis of span{B, A B, ..., A^(k-1) B}."""
+    blocks, Y = [], Bop
+    for _ in range(k):
+        blocks.append(Y)
+        Y = Aop @ Y
+    V, _ = np.linalg.qr(np.concatenate(blocks, axis=1))
+    return V
+
+
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_cocg_rq_iterates_are_block_galerkin(orc, jacobi):
+    """Block COCG (the reference's bl_cocg_rq recurrences,
+    bl_cocg_rq_jacobi_mvp_fft.cpp:41-75) on a complex-symmetric A: the k-th
+    iterate is fixed by the bilinear Galerkin condition V^T (B - A X_k) = 0
+    on V = span{B, AB, ..., A^(k-1) B}, i.e. X_k = V (V^T A V)^(-1) V^T B,
+    and its residual blocks are mutually bilinearly orthogonal,
+    R_i^T R_j = 0 (i != j).  A wrong alpha, beta, tau or xi term breaks both.
+    (DIAG is a scalar here, so the Jacobi operator is complex symmetric too.)"""
+    n, addr, diag, Au, B, A = _problem(orc, 3)
+    Aop, Bop = (A / diag[:, None], B / diag[:, None]) if jacobi else (A, B)
+    assert np.allclose(Aop, Aop.T, atol=1e-12 * np.abs(Aop).max())
+    R = [Bop]
+    for k in range(1, 5):
+        X, it, _ = orc.solve("cocg_rq", n, 3, addr, Au, diag, B, 1e-30, k, jacobi=jacobi)
+        assert it == k
+        V = _krylov_basis(Aop, Bop, k)
+        Xg = V @ np.linalg.solve(V.T @ Aop @ V, V.T @ Bop)
+        assert np.linalg.norm(X - Xg) / np.linalg.norm(Xg) <= 1e-9, k
+        R.append(Bop - Aop @ X)
+    for i in range(len(R)):
+        for j in range(i + 1, len(R)):
+            G = R[i].T @ R[j]
+            assert np.abs(G).max() <= 1e-8 * np.linalg.norm(R[i]) * np.linalg.norm(R[j]), (i, j)
+
+
+@pytest.mark.parametrize("method", ["bicgstab", "bicggr"])
+@pytest.mark.parametrize("f", [3, 6])
+def test_bicg_iterates_in_block_krylov_space(orc, method, f):
+    """Block BiCGSTAB and BiCGGR use two matvecs per iteration
+    (bl_bicgstab_mvp_fft.cpp:43-84, bl_bicggr_mvp_fft.cpp:57-106), so the
+    k-th iterate lies in span{B, AB, ..., A^(2k-1) B} and, in general, not in
+    the space one power smaller."""
+    n, addr, diag, Au, B, A = _problem(orc, f)
+    for k in (1, 2):
+        X, it, _ = orc.solve(method, n, f, addr, Au, diag, B, 1e-30, k)
+        assert it == k
+        V = _krylov_basis(A, B, 2 * k)
+        assert np.linalg.norm(X - V @ (V.conj().T @ X)) <= 1e-10 * np.linalg.norm(X), k
+        W = _krylov_basis(A, B, 2 * k - 1)
+        assert np.linalg.norm(X - W @ (W.conj().T @ X)) >= 1e-4 * np.linalg.norm(X), k
+
+
 @pytest.mark.parametrize("method,ref", [("cocg_rq", _cocg), ("bicgstab", _bicgstab), ("bicggr", _bicggr)])
 def test_single_rhs_reduces_to_textbook(orc, method, ref):
     n, addr, diag, Au, B, A = _problem(orc, 3, L=1, seed=5)
@@ -221,7 +310,10 @@ def test_single_rhs_reduces_to_textbook(orc, method, ref):
     # system, so the histories are compared where rounding is still small
     m = 6
     assert np.allclose(hist[:m], hr[:m], rtol=1e-10, atol=0)
-    assert abs(len(hist) - len(hr)) <= 0.1 * len(hr)
+    # the late iterations of BiCGSTAB here move with the FFT's rounding by
+    # up to about 15 % of the ~130 steps; both must still converge
+    assert abs(len(hist) - len(hr)) <= 0.2 * len(hr)
+    assert hist[-1] < 1e-9 and hr[-1] < 1e-9
     assert np.linalg.norm(X[:, 0] - xr) / np.linalg.norm(xr) < 1e-7
 
 
