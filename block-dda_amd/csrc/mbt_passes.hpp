@@ -357,24 +357,23 @@ void apply_impl(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int6
     mbt_ensure_workspace(op, sizeof(cplx<R>) * (nU + nV));
     cplx<R>* U = (cplx<R>*)op->ws;
     cplx<R>* V = U + nU;
-    // the planned kernels serve f = 3; `if constexpr` keeps f = 6 from
-    // instantiating them
-    bool planned = false;
-    if constexpr (F == 3) planned = !force_lds_passes();
+    // The planned z and y kernels serve f = 3 and f = 6.  The planned fused x
+    // kernel serves f = 3 only: with six components between the transforms
+    // its f = 6 instances need 160-256 VGPRs (FP64: 256 and spills), one to
+    // three waves per SIMD, so f = 6 keeps the all-in-LDS x kernel.
+    const bool planned = !force_lds_passes();
     // Occupied vectors: only the (ix, iy) lines that hold an occupied cell
     // are transformed along z.  The y inverse pass skips the U rows of the
     // other lines and the z inverse pass skips those lines, since nothing is
     // gathered from them.  The z forward pass skips their rows only when the
     // planned y forward pass runs with the line mask, which reads them as zero.
     const bool lined = c2o && op->lines;
-    bool ymask = false;
-    if constexpr (F == 3) ymask = planned && lined && lines_plan_ok<R>(d.Py, Pzs);
+    const bool ymask = planned && lined && lines_plan_ok<R>(d.Py, Pzs);
     const int* zl = ymask ? op->lines : nullptr;
     const bool tm = op->timing != 0;
     if (tm) BDDA_HIP(hipEventRecord(op->ev[0], st));
     // z forward + scatter: X -> U
-    bool done = false;
-    if constexpr (F == 3) done = planned && launch_zfwd_plan<R, F>(X, ldx, c2o, U, d, Pzs, tw[2], s, st, zl, op->nlines);
+    bool done = planned && launch_zfwd_plan<R, F>(X, ldx, c2o, U, d, Pzs, tw[2], s, st, zl, op->nlines);
     if (!done) {
         const int P = d.Pz;
         if ((int64_t)P * F > (int64_t)Budget<R>::MAXE * NT_PASS) throw HipError{hipErrorInvalidValue, "Pz too long"};
@@ -388,9 +387,7 @@ void apply_impl(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int6
     if (tm) BDDA_HIP(hipEventRecord(op->ev[1], st));
     // y forward: U[outer = (rhs F + c) nx + ix][ny][Pzs] -> V[outer][Py][Pzs]
     const int64_t outer = (int64_t)s * F * d.nx;
-    done = false;
-    if constexpr (F == 3)
-        done = planned && launch_lines_plan<R, false>(U, V, d.ny, d.Py, d.Pz, Pzs, outer, d.Py, tw[1], st,
+    done = planned && launch_lines_plan<R, false>(U, V, d.ny, d.Py, d.Pz, Pzs, outer, d.Py, tw[1], st,
                                                       ymask ? op->lmask : nullptr, d.nx);
     if (!done) launch_lines<R, false>(U, V, d.ny, d.Py, d.Pz, Pzs, outer, op->ax[1], tw[1], st);
     if (tm) BDDA_HIP(hipEventRecord(op->ev[2], st));
@@ -411,16 +408,12 @@ void apply_impl(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int6
     }
     if (tm) BDDA_HIP(hipEventRecord(op->ev[3], st));
     // y inverse: V -> U keeping ny
-    done = false;
-    if constexpr (F == 3)
-        done = planned && launch_lines_plan<R, true>(V, U, d.Py, d.ny, d.Pz, Pzs, outer, d.Py, tw[1], st,
+    done = planned && launch_lines_plan<R, true>(V, U, d.Py, d.ny, d.Pz, Pzs, outer, d.Py, tw[1], st,
                                                      lined ? op->lmask : nullptr, d.nx);
     if (!done) launch_lines<R, true>(V, U, d.Py, d.ny, d.Pz, Pzs, outer, op->ax[1], tw[1], st);
     if (tm) BDDA_HIP(hipEventRecord(op->ev[4], st));
     // z inverse + gather: U -> Y
-    done = false;
-    if constexpr (F == 3)
-        done = planned && launch_zinv_plan<R, F>(U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw[2], s, st,
+    done = planned && launch_zinv_plan<R, F>(U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw[2], s, st,
                                                  lined ? op->lines : nullptr, op->nlines);
     if (!done) {
         const int P = d.Pz;

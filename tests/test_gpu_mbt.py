@@ -78,13 +78,15 @@ def test_occupied_matvec_vs_oracle(bdda, orc, f):
     assert rel(Yp, ref.reshape(addr.size, f, 3)[perm].reshape(-1, 3)) <= TOL64
 
 
+@pytest.mark.parametrize("f", [3, 6])
 @pytest.mark.parametrize("jac", [False, True])
-def test_occupied_matvec_sparse_lines(bdda, orc, jac):
+def test_occupied_matvec_sparse_lines(bdda, orc, jac, f):
     """Occupied matvec on the planned passes (14x16x16 -> 32^3) when most
     (ix, iy) lines are empty: the passes then transform only the occupied
     lines.  Occupied lines include the first and the last line, a line whose
-    only cell is its last z, and a full line; the rest are random."""
-    print("case sparse lines jacobi", jac, flush=True)
+    only cell is its last z, and a full line; the rest are random.  f = 6
+    runs the planned z and y passes with the all-in-LDS x pass."""
+    print("case sparse lines jacobi", jac, "f", f, flush=True)
     rng = np.random.default_rng(31)
     n = (14, 16, 16)
     nxny, nz = n[0] * n[1], n[2]
@@ -98,7 +100,6 @@ def test_occupied_matvec_sparse_lines(bdda, orc, jac):
         else:
             cells.extend(l * nz + rng.choice(nz, size=int(rng.integers(1, nz)), replace=False))
     addr = np.array(sorted(cells), dtype=np.int32)
-    f = 3
     alpha = rng.standard_normal(addr.size) * 0.1 + 0.3 + 0.05j
     diag = np.repeat(1.0 / alpha, f)
     lf, k = 0.4, 1.7
@@ -121,15 +122,15 @@ def test_occupied_matvec_sparse_lines(bdda, orc, jac):
     assert rel(op.apply_occupied(X, "fp64", jac), ref) <= TOL64
 
 
+@pytest.mark.parametrize("f", [3, 6])
 @pytest.mark.parametrize("n", [(14, 16, 10), (14, 10, 16)])
-def test_occupied_lines_mixed_plans(bdda, orc, n):
+def test_occupied_lines_mixed_plans(bdda, orc, n, f):
     """Occupied matvec where only one of the y and z axes has a planned size
     (14x16x10 -> 32x32x20: planned y passes with line flags, all-in-LDS z
     passes over every line; 14x10x16 -> 32x20x32: planned z passes over the
     occupied lines, all-in-LDS y passes) against the oracle."""
-    print("case mixed plans", n, flush=True)
+    print("case mixed plans", n, "f", f, flush=True)
     rng = np.random.default_rng(43)
-    f = 3
     nxny, nz = n[0] * n[1], n[2]
     lines = np.sort(rng.choice(nxny, size=nxny // 3, replace=False))
     addr = np.concatenate([l * nz + np.flatnonzero(rng.random(nz) < 0.5) for l in lines]).astype(np.int32)
