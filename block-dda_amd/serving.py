@@ -273,6 +273,9 @@ class Server:
         if not ok:
             return {"requests": len(self.completed), "failed": len(self.completed)}
         lat = np.array([r.latency for r in ok])
+        # service time: from the start of a request's batch to its finish,
+        # without the time it queued (latency = queueing + service)
+        svc = np.array([r.finish - r.start for r in ok])
         t0 = min(r.arrival for r in self.completed)
         t1 = max(r.finish for r in self.completed)
         span = max(t1 - t0, 1e-9)
@@ -280,6 +283,7 @@ class Server:
             "requests": len(self.completed), "failed": len(self.completed) - len(ok),
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "max_s": float(lat.max()),
+            "service_p50_s": float(np.percentile(svc, 50)), "service_p99_s": float(np.percentile(svc, 99)),
             "solves_per_s": len(ok) / span,
             "rhs_per_s": sum(r.cols for r in ok) / span,
             "batches": len(self.batches),
