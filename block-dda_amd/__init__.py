@@ -81,6 +81,7 @@ SIGNATURES = {
                                   ctypes.c_double, ctypes.c_int, _vp, _i64, ctypes.POINTER(SolveInfo), _dp,
                                   ctypes.c_int]),
     "bdda_qr_reduced": (ctypes.c_int, [_vp, _i64, ctypes.c_int, _vp, _vp]),
+    "bdda_lu_solve": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "bdda_dev_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "bdda_dev_free": (ctypes.c_int, [_vp]),
     "bdda_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
@@ -338,6 +339,18 @@ bl_bicgstab_mvp_fft = _solver(BICGSTAB, False)
 bl_bicgstab_jacobi_mvp_fft = _solver(BICGSTAB, True)
 bl_bicggr_mvp_fft = _solver(BICGGR, False)
 bl_bicggr_jacobi_mvp_fft = _solver(BICGGR, True)
+
+
+def lu_solve(A, B, full=False):
+    """X = A^{-1} B on the GPU (the block solvers' s x s solves: Eigen
+    PartialPivLU / FullPivLU .solve(), bl_cocg_rq_mvp_fft.cpp:56,66)."""
+    A = _cz(A)
+    B = _cz(B)
+    s = A.shape[0]
+    B2 = B.reshape(s, -1, order="F")
+    X = np.zeros_like(B2, order="F")
+    _check(lib().bdda_lu_solve(_ptr(A), _ptr(B2), _ptr(X), s, B2.shape[1], int(bool(full))))
+    return X.reshape(B.shape, order="F")
 
 
 def qr_reduced(A):

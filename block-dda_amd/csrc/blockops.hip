@@ -425,11 +425,15 @@ struct ComboDev {
     const cplx<R>* V[3];
     int64_t ldv[3];
     cplx<R> a[3];
+    const c64* ad[3];     // device scalars scaling a (or null)
     int nm;
     const cplx<R>* W[2];
     int64_t ldw[2];
     int kw[2];
     const cplx<R>* M[2];  // device, kw x s column-major, already in precision R
+    const c64* M64[2];    // or device FP64 (converted while staged), scaled by ms (and *msd)
+    c64 ms[2];
+    const c64* msd[2];
 };
 
 // Z[:, j0 + jj] = sum_t a_t V_t[:, j0+jj] + sum_u sum_l W_u[:, l] M_u[l, j0+jj]
@@ -443,12 +447,40 @@ __global__ __launch_bounds__(256) void k_combo(ComboDev<R> c, cplx<R>* Z, int64_
 {
     __shared__ cplx<R> Ms[2][KW > 0 ? KW : 1][SJ];
     if constexpr (KW > 0) {
-        for (int u = 0; u < c.nm; ++u)
-            for (int e = threadIdx.x; e < KW * SJ; e += blockDim.x) {
-                const int l = e / SJ, jj = e % SJ;
-                Ms[u][l][jj] = (l < c.kw[u] && jj < nj) ? c.M[u][l + (int64_t)(j0 + jj) * c.kw[u]] : czero<R>();
+        for (int u = 0; u < c.nm; ++u) {
+            if (c.M64[u]) {
+                c64 sc = c.ms[u];
+                if (c.msd[u]) {
+                    const c64 d = *c.msd[u];
+                    sc = mk(sc.x * d.x - sc.y * d.y, sc.x * d.y + sc.y * d.x);
+                }
+                for (int e = threadIdx.x; e < KW * SJ; e += blockDim.x) {
+                    const int l = e / SJ, jj = e % SJ;
+                    c64 v = mk(0.0, 0.0);
+                    if (l < c.kw[u] && jj < nj) {
+                        const c64 m = c.M64[u][l + (int64_t)(j0 + jj) * c.kw[u]];
+                        v = mk(sc.x * m.x - sc.y * m.y, sc.x * m.y + sc.y * m.x);
+                    }
+                    Ms[u][l][jj] = mk((R)v.x, (R)v.y);
+                }
+            } else {
+                for (int e = threadIdx.x; e < KW * SJ; e += blockDim.x) {
+                    const int l = e / SJ, jj = e % SJ;
+                    Ms[u][l][jj] = (l < c.kw[u] && jj < nj) ? c.M[u][l + (int64_t)(j0 + jj) * c.kw[u]] : czero<R>();
+                }
             }
+        }
         __syncthreads();
+    }
+    cplx<R> av[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        av[t] = c.a[t];
+        if (t < c.nv && c.ad[t]) {
+            const c64 d = *c.ad[t];
+            av[t] = mk((R)((double)c.a[t].x * d.x - (double)c.a[t].y * d.y),
+                       (R)((double)c.a[t].x * d.y + (double)c.a[t].y * d.x));
+        }
     }
     // one row per thread: a grid-stride loop would let the compiler hoist
     // the whole LDS coefficient block into registers
@@ -465,7 +497,7 @@ __global__ __launch_bounds__(256) void k_combo(ComboDev<R> c, cplx<R>* Z, int64_
 #pragma unroll
                 for (int jj = 0; jj < SJ; ++jj) v[jj] = (jj < nj) ? V[(int64_t)jj * c.ldv[t]] : czero<R>();
 #pragma unroll
-                for (int jj = 0; jj < SJ; ++jj) fma_c(acc[jj], c.a[t], v[jj]);
+                for (int jj = 0; jj < SJ; ++jj) fma_c(acc[jj], av[t], v[jj]);
             }
         if constexpr (KW > 0) {
 #pragma unroll
@@ -589,9 +621,11 @@ static HostMat finish_gram(BlockWork& w, int nblk, int np)
     return out;
 }
 
+// launches the partial-sum kernel of a Gram product; returns the number of
+// groups whose a x b partials sit in w.partial
 template <typename R>
-HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* B, int64_t ldb, int b,
-             int64_t rows, int conjA)
+static int gram_launch(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* B, int64_t ldb, int b,
+                       int64_t rows, int conjA)
 {
     const int np = a * b;
     const int ntile = ((a + 3) / 4) * ((b + 3) / 4);
@@ -615,7 +649,7 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
             else launch(k_gram_mfma<R, false, false>);
         }
         BDDA_HIP(hipGetLastError());
-        return finish_gram(w, nblk, np);
+        return nblk;
     }
     if (a <= 64 && b <= 64 && !force_valu_gram()) {
         // wide blocks on the matrix cores; at most 256 groups keep the
@@ -645,7 +679,7 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
         }
 #undef BDDA_GRAM_WIDE
         BDDA_HIP(hipGetLastError());
-        return finish_gram(w, nblk, np);
+        return nblk;
     }
     const size_t c = sizeof(cplx<R>);
     const int pitch = 4 * ((a + 3) / 4) + 4 * ((b + 3) / 4) + 2;
@@ -663,7 +697,52 @@ HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* 
         hipLaunchKernelGGL((k_gram_partial<R, false>), dim3(nblk), dim3(NT_RED), shm, w.stream, A, lda, a, B, ldb, b,
                            rows, TR, rpb, w.partial);
     BDDA_HIP(hipGetLastError());
-    return finish_gram(w, nblk, np);
+    return nblk;
+}
+
+template <typename R>
+HostMat gram(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* B, int64_t ldb, int b,
+             int64_t rows, int conjA)
+{
+    const int nblk = gram_launch<R>(w, A, lda, a, B, ldb, b, rows, conjA);
+    return finish_gram(w, nblk, a * b);
+}
+
+template <typename R>
+void gram_dev(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* B, int64_t ldb, int b, int64_t rows,
+              int conjA, c64* out)
+{
+    const int nblk = gram_launch<R>(w, A, lda, a, B, ldb, b, rows, conjA);
+    const int np = a * b;
+    hipLaunchKernelGGL(k_reduce_partial, dim3((unsigned)ceil_div(np, RED_WAVES)), dim3(64 * RED_WAVES), 0, w.stream,
+                       (const c64*)w.partial, nblk, np, out);
+    BDDA_HIP(hipGetLastError());
+}
+
+template <typename R>
+static int dot_launch(BlockWork& w, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb, int64_t rows, int s,
+                      int conjA)
+{
+    const int nblk = (int)std::min<int64_t>(MAX_PARTIAL_BLOCKS, std::max<int64_t>(1, ceil_div(rows, NT_RED)));
+    w.reserve(nblk, 1);
+    if (conjA)
+        hipLaunchKernelGGL((k_dot_partial<R, true>), dim3(nblk), dim3(NT_RED), 0, w.stream, A, lda, B, ldb, rows, s,
+                           w.partial);
+    else
+        hipLaunchKernelGGL((k_dot_partial<R, false>), dim3(nblk), dim3(NT_RED), 0, w.stream, A, lda, B, ldb, rows, s,
+                           w.partial);
+    BDDA_HIP(hipGetLastError());
+    return nblk;
+}
+
+template <typename R>
+void dot_all_dev(BlockWork& w, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb, int64_t rows, int s,
+                 int conjA, c64* out)
+{
+    const int nblk = dot_launch<R>(w, A, lda, B, ldb, rows, s, conjA);
+    hipLaunchKernelGGL(k_reduce_partial, dim3(1), dim3(64 * RED_WAVES), 0, w.stream, (const c64*)w.partial, nblk, 1,
+                       out);
+    BDDA_HIP(hipGetLastError());
 }
 
 template <typename R>
@@ -747,9 +826,25 @@ void combo(BlockWork& w, const Combo<R>& c, cplx<R>* Z, int64_t ldz, int64_t row
         cd.V[t] = c.V[t];
         cd.ldv[t] = c.ldv[t];
         cd.a[t] = mk((R)c.a[t].x, (R)c.a[t].y);
+        cd.ad[t] = c.ad[t];
     }
     cd.nm = c.nm;
-    if (c.nm > 0) {
+    bool host_m = false;
+    for (int u = 0; u < c.nm; ++u) {
+        if (c.kw[u] > 64 || s > 64) throw HipError{hipErrorInvalidValue, "combo: block wider than 64"};
+        host_m = host_m || c.Md[u] == nullptr;
+    }
+    if (c.nm > 0 && !host_m) {
+        // device coefficients: nothing to stage
+        for (int u = 0; u < c.nm; ++u) {
+            cd.W[u] = c.W[u];
+            cd.ldw[u] = c.ldw[u];
+            cd.kw[u] = c.kw[u];
+            cd.M64[u] = c.Md[u];
+            cd.ms[u] = c.ms[u];
+            cd.msd[u] = c.msd[u];
+        }
+    } else if (c.nm > 0) {
         // stage the coefficient matrices (converted to R) through a rotating
         // pinned slot; wrap-around waits for the stream so a slot is never
         // rewritten while its copy may still be pending
@@ -993,6 +1088,36 @@ int qr_fast(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq
     return BDDA_OK;
 }
 
+template <typename R>
+void qr_fast_dev(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s, c64* Rd,
+                 cplx<R>* scratch, c64* tmp, int* flag)
+{
+    // the same two sweeps as qr_fast, with the Gram, the Cholesky factors and
+    // their inverses on the device: tmp = [G | U1 | I1 | U2], I2 reuses G
+    const double floor1 = (sizeof(R) == 8) ? 1e-12 : 1e-9;
+    const size_t ss = (size_t)s * s;
+    c64 *G = tmp, *U1 = tmp + ss, *I1 = tmp + 2 * ss, *U2 = tmp + 3 * ss;
+    gram_dev<R>(w, A, lda, s, A, lda, s, rows, 1, G);
+    dm_chol_inv(w.stream, G, U1, I1, s, floor1, flag, BDDA_SM_CHOL);
+    Combo<R> c1;
+    c1.nm = 1;
+    c1.W[0] = A;
+    c1.ldw[0] = lda;
+    c1.kw[0] = s;
+    c1.Md[0] = I1;
+    combo<R>(w, c1, scratch, rows, rows, s);
+    gram_dev<R>(w, scratch, rows, s, scratch, rows, s, rows, 1, G);
+    dm_chol_inv(w.stream, G, U2, G, s, 1e-3, flag, BDDA_SM_CHOL);
+    Combo<R> c2;
+    c2.nm = 1;
+    c2.W[0] = scratch;
+    c2.ldw[0] = rows;
+    c2.kw[0] = s;
+    c2.Md[0] = G;
+    combo<R>(w, c2, Q, ldq, rows, s);
+    dm_mul(w.stream, U2, 0, U1, Rd, s, s, s, mk(1.0, 0.0), nullptr, nullptr, mk(0.0, 0.0));
+}
+
 // ------------------------------------------------------------- host algebra
 // The s x s algebra below runs on split real / imaginary planes, column by
 // column over contiguous rows, so the compiler vectorises the inner loops
@@ -1190,7 +1315,13 @@ bool hm_lu_solve(const HostMat& A_, const HostMat& B, int s, int nb, int full, H
     template int qr_cgs2<R>(BlockWork&, const cplx<R>*, int64_t, cplx<R>*, int64_t, int64_t, int, HostMat&);   \
     template int qr_mgs2<R>(BlockWork&, const cplx<R>*, int64_t, cplx<R>*, int64_t, int64_t, int, HostMat&);   \
     template int qr_fast<R>(BlockWork&, const cplx<R>*, int64_t, cplx<R>*, int64_t, int64_t, int, HostMat&, \
-                            cplx<R>*);
+                            cplx<R>*);                                                                          \
+    template void qr_fast_dev<R>(BlockWork&, const cplx<R>*, int64_t, cplx<R>*, int64_t, int64_t, int, c64*,    \
+                                 cplx<R>*, c64*, int*);                                                         \
+    template void gram_dev<R>(BlockWork&, const cplx<R>*, int64_t, int, const cplx<R>*, int64_t, int, int64_t,  \
+                              int, c64*);                                                                       \
+    template void dot_all_dev<R>(BlockWork&, const cplx<R>*, int64_t, const cplx<R>*, int64_t, int64_t, int, int, \
+                                 c64*);
 BDDA_INST(float)
 BDDA_INST(double)
 

@@ -44,7 +44,9 @@ double norm_fro(BlockWork& w, const cplx<R>* A, int64_t lda, int64_t rows, int s
 // no NaN or Inf among the n values of A (device, FP64 complex)
 bool all_finite(BlockWork& w, const c64* A, int64_t n);
 
-// Z = sum_t a_t V_t + sum_u W_u M_u  (M_u: host FP64, k_u x s, column-major).
+// Z = sum_t a_t V_t + sum_u W_u M_u  (M_u: FP64, k_u x s, column-major, on the
+// host (M) or on the device (Md, scaled by ms_u and, when msd_u is given, by the
+// device scalar *msd_u); a_t is scaled by the device scalar *ad_t when given).
 // Z may alias a V_t but never a W_u.
 template <typename R>
 struct Combo {
@@ -52,11 +54,15 @@ struct Combo {
     const cplx<R>* V[3];
     int64_t ldv[3];
     c64 a[3];
+    const c64* ad[3] = {nullptr, nullptr, nullptr};
     int nm = 0;
     const cplx<R>* W[2];
     int64_t ldw[2];
     int kw[2];
-    const c64* M[2];  // host pointers
+    const c64* M[2] = {nullptr, nullptr};   // host pointers
+    const c64* Md[2] = {nullptr, nullptr};  // device pointers
+    c64 ms[2] = {{1.0, 0.0}, {1.0, 0.0}};
+    const c64* msd[2] = {nullptr, nullptr};
 };
 template <typename R>
 void combo(BlockWork& w, const Combo<R>& c, cplx<R>* Z, int64_t ldz, int64_t rows, int s);
@@ -97,6 +103,44 @@ int qr_mgs2(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq
 template <typename R>
 int qr_fast(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s,
             HostMat& Rm, cplx<R>* scratch);
+
+// ---------------- s x s algebra on the device (smallmat.hip): FP64, s <= 64,
+// one work-group per call, nothing returns to the host.  Failures set a bit of
+// the device flag word: BDDA_SM_LU / BDDA_SM_LU2 (exactly zero LU pivot of the
+// solve the caller tags with it; X is then zero) or BDDA_SM_CHOL (Cholesky
+// pivot below its floor).  After an LU breakdown every later LU solve and
+// scalar quotient returns zero, so the block updates stop.
+constexpr int BDDA_SM_LU = 1, BDDA_SM_CHOL = 2, BDDA_SM_LU2 = 4, BDDA_SM_LU_ANY = BDDA_SM_LU | BDDA_SM_LU2;
+// X (s x nb) = A^{-1} (bscale B [/ *bdiv]), partial (full = 0) or full
+// pivoting; X may alias B
+void dm_lu_solve(hipStream_t st, const c64* A, const c64* B, c64* X, int s, int nb, int full, c64 bscale,
+                 const c64* bdiv, int* flag, int bit);
+// C (m x n) = alpha [*alpha_dev] op(A) B [+ beta C0]; op(A) = A (m x k) or A^T (A: k x m)
+void dm_mul(hipStream_t st, const c64* A, int transA, const c64* B, c64* C, int m, int k, int n, c64 alpha,
+            const c64* alpha_dev, const c64* C0, c64 beta);
+// U = chol_upper(G) (G = U^H U), Ui = U^{-1}; pivot d <= rel_floor G_jj fails
+void dm_chol_inv(hipStream_t st, const c64* G, c64* U, c64* Ui, int s, double rel_floor, int* flag, int bit);
+// *out = ||A||_F over n entries
+void dm_fro(hipStream_t st, const c64* A, int n, double* out);
+// *out = mult * (*num) / (*den), or zero after an LU breakdown
+void dm_scalar_div(hipStream_t st, const c64* num, const c64* den, c64 mult, c64* out, const int* flag);
+
+// Gram C (a x b) = op(A)^ B and the sum of conj(A) .* B (conjA) or A .* B,
+// reduced in FP64 into device memory (stream-ordered, no host wait)
+template <typename R>
+void gram_dev(BlockWork& w, const cplx<R>* A, int64_t lda, int a, const cplx<R>* B, int64_t ldb, int b, int64_t rows,
+              int conjA, c64* out);
+template <typename R>
+void dot_all_dev(BlockWork& w, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb, int64_t rows, int s,
+                 int conjA, c64* out);
+
+// qr_fast with its s x s algebra on the device: R (s x s) to device memory,
+// tmp holds 4 s x s matrices.  A Cholesky pivot below its floor sets
+// BDDA_SM_CHOL in *flag (Q and R are then undefined; the caller redoes the
+// factorisation with qr_fast on the host path).
+template <typename R>
+void qr_fast_dev(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s, c64* Rd,
+                 cplx<R>* scratch, c64* tmp, int* flag);
 
 // ---------------- small dense FP64 algebra on the host (s <= 64)
 HostMat hm_mul(const HostMat& A, const HostMat& B, int m, int k, int n);       // A (m x k) B (k x n)

@@ -597,6 +597,40 @@ int bdda_qr_reduced(const bdda_c64* A, int64_t m, int n, bdda_c64* Q, bdda_c64* 
     })
 }
 
+int bdda_lu_solve(const bdda_c64* A, const bdda_c64* B, bdda_c64* X, int s, int nb, int full)
+{
+    if (!A || (nb > 0 && (!B || !X))) return fail(BDDA_EINVAL, "null pointer");
+    if (s < 1 || s > 64 || nb < 0 || nb > 64) return fail(BDDA_EINVAL, "need 1 <= s <= 64 and 0 <= nb <= 64");
+    BDDA_TRY({
+        hipStream_t st;
+        BDDA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        struct S {
+            hipStream_t s;
+            ~S() { (void)hipStreamDestroy(s); }
+        } sg{st};
+        const size_t na = (size_t)s * s, nx = (size_t)s * nb;
+        c64* d = nullptr;
+        BDDA_HIP(hipMalloc(&d, sizeof(c64) * (na + 2 * nx + 1)));
+        struct Free {
+            void* p;
+            ~Free() { if (p) (void)hipFree(p); }
+        } fd{d};
+        c64 *dA = d, *dB = d + na, *dX = dB + nx;
+        int* flag = reinterpret_cast<int*>(dX + nx);
+        BDDA_HIP(hipMemcpy(dA, A, sizeof(c64) * na, hipMemcpyHostToDevice));
+        if (nx) BDDA_HIP(hipMemcpy(dB, B, sizeof(c64) * nx, hipMemcpyHostToDevice));
+        const int zero = 0;
+        BDDA_HIP(hipMemcpy(flag, &zero, sizeof(int), hipMemcpyHostToDevice));
+        dm_lu_solve(st, dA, dB, dX, s, nb, full ? 1 : 0, mk(1.0, 0.0), nullptr, flag, BDDA_SM_LU);
+        int f = 0;
+        BDDA_HIP(hipMemcpyAsync(&f, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        BDDA_HIP(hipStreamSynchronize(st));
+        if (f) return fail(BDDA_EBREAKDOWN, "lu_solve: exactly zero pivot");
+        if (nx) BDDA_HIP(hipMemcpy(X, dX, sizeof(c64) * nx, hipMemcpyDeviceToHost));
+        return BDDA_OK;
+    })
+}
+
 int bdda_dev_alloc(size_t bytes, void** p)
 {
     if (!p) return fail(BDDA_EINVAL, "null pointer");

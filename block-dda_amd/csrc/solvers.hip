@@ -6,8 +6,11 @@
 //   bl_bicgstab_mvp_fft.cpp:21-94, bl_bicgstab_jacobi_mvp_fft.cpp:20-95
 //   bl_bicggr_mvp_fft.cpp:18-113,  bl_bicggr_jacobi_mvp_fft.cpp:18-116
 // Block vectors live on the device (FP32 or FP64); Gram matrices, traces and
-// norms are reduced in FP64 and the s x s algebra (LU, products) runs in
-// FP64 on the host exactly where the reference calls Eigen.
+// norms are reduced in FP64, and the s x s algebra (LU, products, the
+// Cholesky factors of the QR) runs in FP64 on the device (smallmat.hip)
+// exactly where the reference calls Eigen.  An iteration is enqueued
+// without waiting on the host; the host reads one value per iteration, the
+// residual norm of the convergence test, with the breakdown flags.
 //
 // Documented differences (equal in exact arithmetic):
 //   - qr_reduced inside the loops is Cholesky-QR applied twice with a
@@ -18,6 +21,7 @@
 #include "solvers.hpp"
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 
@@ -96,26 +100,6 @@ void copy_block(BlockWork& w, const cplx<R>* A, cplx<R>* Z, int64_t rows, int s)
     BDDA_HIP(hipMemcpyAsync(Z, A, sizeof(cplx<R>) * rows * s, hipMemcpyDeviceToDevice, w.stream));
 }
 
-inline HostMat neg(const HostMat& A)
-{
-    HostMat B(A.size());
-    for (size_t i = 0; i < A.size(); ++i) B[i] = mk(-A[i].x, -A[i].y);
-    return B;
-}
-
-inline HostMat scal(const HostMat& A, c64 a)
-{
-    HostMat B(A.size());
-    for (size_t i = 0; i < A.size(); ++i) B[i] = a * A[i];
-    return B;
-}
-
-inline c64 cdiv(c64 a, c64 b)
-{
-    const double den = b.x * b.x + b.y * b.y;
-    return mk((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
-}
-
 struct Ctx {
     Problem& P;
     int jacobi;
@@ -125,6 +109,63 @@ struct Ctx {
     std::vector<double>* hist;
 };
 
+
+
+
+// ------------------------------------------------------------ device s x s state
+// The solvers below keep every s x s quantity (Grams, LU solutions, the QR's
+// R, traces) in device memory and enqueue a whole iteration without a host
+// round trip; the host reads one probe per iteration (the residual norm the
+// convergence test needs and the breakdown flags).
+struct SmallState {
+    BlockWork& w;
+    int s;
+    DevBuf<c64> buf;
+    c64* probe;  // [0].x: the value read per iteration; [1]: flag word
+    int* flag;
+    SmallState(BlockWork& w_, int s_, int nmat, int64_t rows) : w(w_), s(s_), buf((size_t)nmat * s_ * s_ + 4)
+    {
+        probe = buf.p + (size_t)nmat * s * s;
+        flag = reinterpret_cast<int*>(probe + 1);
+        // partial sums for every Gram and dot of the solve, reserved up front:
+        // a reallocation would free memory that queued kernels still use
+        const size_t np = (size_t)s * s;
+        w.reserve(std::max<size_t>(1024, (s <= 16 ? 1024 : 256) * np), std::max<size_t>(np, 1));
+        (void)rows;
+        fill_zero<double>(w.stream, probe, 2, 2, 1);
+    }
+    c64* m(int i) { return buf.p + (size_t)i * s * s; }
+    struct Read {
+        double v;
+        int flags;
+    };
+    Read read()
+    {
+        BDDA_HIP(hipMemcpyAsync(w.host_pinned + w.host_elems, probe, 2 * sizeof(c64), hipMemcpyDeviceToHost,
+                                w.stream));
+        BDDA_HIP(hipStreamSynchronize(w.stream));
+        Read r;
+        r.v = w.host_pinned[w.host_elems].x;
+        std::memcpy(&r.flags, &w.host_pinned[w.host_elems + 1], sizeof(int));
+        return r;
+    }
+    // host matrix -> device (stream idle first: the solvers call this only
+    // at set-up and on the Cholesky fallback)
+    void upload(c64* dst, const HostMat& h)
+    {
+        BDDA_HIP(hipStreamSynchronize(w.stream));
+        BDDA_HIP(hipMemcpy(dst, h.data(), sizeof(c64) * h.size(), hipMemcpyHostToDevice));
+    }
+    void clear_flag(int bits)
+    {
+        int f = 0;
+        BDDA_HIP(hipStreamSynchronize(w.stream));
+        BDDA_HIP(hipMemcpy(&f, flag, sizeof(int), hipMemcpyDeviceToHost));
+        f &= ~bits;
+        BDDA_HIP(hipMemcpy(flag, &f, sizeof(int), hipMemcpyHostToDevice));
+    }
+};
+
 // ------------------------------------------------------------ COCG-rq
 template <typename R>
 int run_cocg_rq(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
@@ -132,6 +173,7 @@ int run_cocg_rq(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     BlockWork& w = c.P.work;
     const int s = c.s;
     const int64_t rows = c.rows, N = rows * s;
+    const c64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
     DevBuf<cplx<R>> Bp(c.jacobi ? N : 0), Q(N), Qn(N), S(N), AS(N), T(N), scr(N);
     const cplx<R>* Bu = B;
     if (c.jacobi) {  // B_pre = jpre .* B  (bl_cocg_rq_jacobi_mvp_fft.cpp:22-27)
@@ -140,12 +182,16 @@ int run_cocg_rq(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     }
     const double bnorm = norm_fro<R>(w, Bu, rows, rows, s);
     fill_zero<R>(w.stream, X, rows, rows, s);
-    HostMat xi, tau;
-    int rc = qr_fast<R>(w, Bu, rows, Q.p, rows, rows, s, xi, scr.p);
+    HostMat xi0;
+    int rc = qr_fast<R>(w, Bu, rows, Q.p, rows, rows, s, xi0, scr.p);
     if (rc) return rc;
     copy_block<R>(w, Q.p, S.p, rows, s);
     const int full = c.jacobi ? 0 : 1;  // FullPivLU (plain) vs PartialPivLU (jacobi)
-    HostMat G2 = gram<R>(w, Q.p, rows, s, Q.p, rows, s, rows, 0);  // Q^T Q
+    SmallState sm(w, s, 14, rows);
+    c64 *G1 = sm.m(0), *G2 = sm.m(1), *G3 = sm.m(2), *alpha = sm.m(3), *axi = sm.m(4), *xi = sm.m(5);
+    c64 *xin = sm.m(6), *tau = sm.m(7), *rhs = sm.m(8), *beta = sm.m(9), *qtmp = sm.m(10);  // qtmp: 4 matrices
+    sm.upload(xi, xi0);
+    gram_dev<R>(w, Q.p, rows, s, Q.p, rows, s, rows, 0, G2);  // Q^T Q
     cplx<R>* pS = S.p;
     cplx<R>* pT = T.p;
     cplx<R>* pQ = Q.p;
@@ -153,27 +199,45 @@ int run_cocg_rq(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     for (int k = 0; k < itermax; ++k) {
         problem_matvec<R>(c.P, pS, AS.p, s, c.jacobi);
         ++c.info.matvecs;
-        HostMat G1 = gram<R>(w, pS, rows, s, AS.p, rows, s, rows, 0);  // S^T AS
-        HostMat alpha;
-        if (!hm_lu_solve(G1, G2, s, s, full, alpha)) return BDDA_EBREAKDOWN;
-        HostMat axi = hm_mul(alpha, xi, s, s, s);
-        {   // X = X + S alpha xi
+        gram_dev<R>(w, pS, rows, s, AS.p, rows, s, rows, 0, G1);  // S^T AS
+        dm_lu_solve(w.stream, G1, G2, alpha, s, s, full, one, nullptr, sm.flag, BDDA_SM_LU);
+        dm_mul(w.stream, alpha, 0, xi, axi, s, s, s, one, nullptr, nullptr, zero);
+        {   // X = X + S (alpha xi)
             Combo<R> cb;
-            cb.nv = 1; cb.V[0] = X; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.nm = 1; cb.W[0] = pS; cb.ldw[0] = rows; cb.kw[0] = s; cb.M[0] = axi.data();
+            cb.nv = 1; cb.V[0] = X; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.nm = 1; cb.W[0] = pS; cb.ldw[0] = rows; cb.kw[0] = s; cb.Md[0] = axi;
             combo<R>(w, cb, X, rows, rows, s);
         }
-        HostMat nalpha = neg(alpha);
         {   // T = Q - AS alpha
             Combo<R> cb;
-            cb.nv = 1; cb.V[0] = pQ; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.nm = 1; cb.W[0] = AS.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.M[0] = nalpha.data();
+            cb.nv = 1; cb.V[0] = pQ; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.nm = 1; cb.W[0] = AS.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.Md[0] = alpha; cb.ms[0] = mk(-1.0, 0.0);
             combo<R>(w, cb, pT, rows, rows, s);
         }
-        rc = qr_fast<R>(w, pT, rows, pQn, rows, rows, s, tau, scr.p);
-        if (rc) return rc;
-        xi = hm_mul(tau, xi, s, s, s);
-        const double err = hm_fro(xi) / bnorm;
+        qr_fast_dev<R>(w, pT, rows, pQn, rows, rows, s, tau, scr.p, qtmp, sm.flag);
+        // xi = tau xi, its norm, and (for the next direction) beta = (Q^T Q)^{-1} tau^T (Qnew^T Qnew)
+        auto tail = [&]() {
+            dm_mul(w.stream, tau, 0, xi, xin, s, s, s, one, nullptr, nullptr, zero);
+            dm_fro(w.stream, xin, s * s, &sm.probe[0].x);
+            gram_dev<R>(w, pQn, rows, s, pQn, rows, s, rows, 0, G3);
+            dm_mul(w.stream, tau, 1, G3, rhs, s, s, s, one, nullptr, nullptr, zero);
+            dm_lu_solve(w.stream, G2, rhs, beta, s, s, full, one, nullptr, sm.flag, BDDA_SM_LU2);
+        };
+        tail();
+        SmallState::Read rd = sm.read();
+        if (rd.flags & BDDA_SM_CHOL) {
+            // a Cholesky pivot of the QR fell below its floor: factor T again
+            // on the host path (Cholesky checks, then column Gram-Schmidt)
+            HostMat tauh;
+            rc = qr_fast<R>(w, pT, rows, pQn, rows, rows, s, tauh, scr.p);
+            if (rc) return rc;
+            sm.upload(tau, tauh);
+            sm.clear_flag(BDDA_SM_CHOL);
+            tail();
+            rd = sm.read();
+        }
+        if (rd.flags & BDDA_SM_LU) return BDDA_EBREAKDOWN;
+        const double err = rd.v / bnorm;
         c.info.iters = k + 1;
         c.info.recurrence_res = err;
         if (c.hist) c.hist->push_back(err);
@@ -182,19 +246,17 @@ int run_cocg_rq(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
             c.info.converged = 1;
             break;
         }
-        HostMat G3 = gram<R>(w, pQn, rows, s, pQn, rows, s, rows, 0);  // Qnew^T Qnew
-        HostMat rhs = hm_tmul(tau, G3, s, s, s);                         // tau^T (Qnew^T Qnew)
-        HostMat beta;
-        if (!hm_lu_solve(G2, rhs, s, s, full, beta)) return BDDA_EBREAKDOWN;
+        if (rd.flags & BDDA_SM_LU2) return BDDA_EBREAKDOWN;
         {   // S = Qnew + S beta
             Combo<R> cb;
-            cb.nv = 1; cb.V[0] = pQn; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.nm = 1; cb.W[0] = pS; cb.ldw[0] = rows; cb.kw[0] = s; cb.M[0] = beta.data();
+            cb.nv = 1; cb.V[0] = pQn; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.nm = 1; cb.W[0] = pS; cb.ldw[0] = rows; cb.kw[0] = s; cb.Md[0] = beta;
             combo<R>(w, cb, pT, rows, rows, s);
         }
         std::swap(pS, pT);
         std::swap(pQ, pQn);
-        G2 = std::move(G3);
+        std::swap(G2, G3);
+        std::swap(xi, xin);
     }
     BDDA_HIP(hipStreamSynchronize(w.stream));
     return BDDA_OK;
@@ -207,6 +269,7 @@ int run_bicgstab(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     BlockWork& w = c.P.work;
     const int s = c.s;
     const int64_t rows = c.rows, N = rows * s;
+    const c64 one = mk(1.0, 0.0);
     DevBuf<cplx<R>> Rb(N), Pb(N), Pn(N), R0(N), V(N), T(N), Z(N);
     if (c.jacobi) scale_rows<R>(w.stream, c.P.jpre<R>(), B, rows, Rb.p, rows, rows, s);
     else copy_block<R>(w, B, Rb.p, rows, s);
@@ -214,41 +277,47 @@ int run_bicgstab(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     fill_zero<R>(w.stream, X, rows, rows, s);
     copy_block<R>(w, Rb.p, Pb.p, rows, s);
     copy_block<R>(w, Rb.p, R0.p, rows, s);
+    SmallState sm(w, s, 6, rows);
+    c64 *M1 = sm.m(0), *M2 = sm.m(1), *M3 = sm.m(2), *alpha = sm.m(3), *beta = sm.m(4), *sc = sm.m(5);
+    c64 *zt = sc, *zz = sc + 1, *qsi = sc + 2;
     cplx<R>* pP = Pb.p;
     cplx<R>* pPn = Pn.p;
     for (int k = 0; k < itermax; ++k) {
         problem_matvec<R>(c.P, pP, V.p, s, c.jacobi);  // V = A P
         ++c.info.matvecs;
-        HostMat M1 = gram<R>(w, R0.p, rows, s, V.p, rows, s, rows, 1);   // R0^H V
-        HostMat M2 = gram<R>(w, R0.p, rows, s, Rb.p, rows, s, rows, 1);  // R0^H R
-        HostMat alpha;
-        if (!hm_lu_solve(M1, M2, s, s, 0, alpha)) return BDDA_EBREAKDOWN;
-        HostMat nalpha = neg(alpha);
+        gram_dev<R>(w, R0.p, rows, s, V.p, rows, s, rows, 1, M1);   // R0^H V
+        gram_dev<R>(w, R0.p, rows, s, Rb.p, rows, s, rows, 1, M2);  // R0^H R
+        dm_lu_solve(w.stream, M1, M2, alpha, s, s, 0, one, nullptr, sm.flag, BDDA_SM_LU);
         {   // T = R - V alpha
             Combo<R> cb;
-            cb.nv = 1; cb.V[0] = Rb.p; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.nm = 1; cb.W[0] = V.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.M[0] = nalpha.data();
+            cb.nv = 1; cb.V[0] = Rb.p; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.nm = 1; cb.W[0] = V.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.Md[0] = alpha; cb.ms[0] = mk(-1.0, 0.0);
             combo<R>(w, cb, T.p, rows, rows, s);
         }
         problem_matvec<R>(c.P, T.p, Z.p, s, c.jacobi);  // Z = A T
         ++c.info.matvecs;
-        const c64 zt = dot_all<R>(w, Z.p, rows, T.p, rows, rows, s, 1);
-        const c64 zz = dot_all<R>(w, Z.p, rows, Z.p, rows, rows, s, 1);
-        const c64 qsi = cdiv(zt, zz);
+        dot_all_dev<R>(w, Z.p, rows, T.p, rows, rows, s, 1, zt);
+        dot_all_dev<R>(w, Z.p, rows, Z.p, rows, rows, s, 1, zz);
+        dm_scalar_div(w.stream, zt, zz, one, qsi, sm.flag);
         {   // X = X + P alpha + qsi T
             Combo<R> cb;
-            cb.nv = 2; cb.V[0] = X; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.V[1] = T.p; cb.ldv[1] = rows; cb.a[1] = qsi;
-            cb.nm = 1; cb.W[0] = pP; cb.ldw[0] = rows; cb.kw[0] = s; cb.M[0] = alpha.data();
+            cb.nv = 2; cb.V[0] = X; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.V[1] = T.p; cb.ldv[1] = rows; cb.a[1] = one; cb.ad[1] = qsi;
+            cb.nm = 1; cb.W[0] = pP; cb.ldw[0] = rows; cb.kw[0] = s; cb.Md[0] = alpha;
             combo<R>(w, cb, X, rows, rows, s);
         }
         {   // R = T - qsi Z
             Combo<R> cb;
-            cb.nv = 2; cb.V[0] = T.p; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.V[1] = Z.p; cb.ldv[1] = rows; cb.a[1] = mk(-qsi.x, -qsi.y);
+            cb.nv = 2; cb.V[0] = T.p; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.V[1] = Z.p; cb.ldv[1] = rows; cb.a[1] = mk(-1.0, 0.0); cb.ad[1] = qsi;
             combo<R>(w, cb, Rb.p, rows, rows, s);
         }
-        const double err = norm_fro<R>(w, Rb.p, rows, rows, s) / bnorm;
+        dot_all_dev<R>(w, Rb.p, rows, Rb.p, rows, rows, s, 1, sm.probe);
+        gram_dev<R>(w, R0.p, rows, s, Z.p, rows, s, rows, 1, M3);  // beta = LU(R0^H V)^{-1} (-R0^H Z)
+        dm_lu_solve(w.stream, M1, M3, beta, s, s, 0, mk(-1.0, 0.0), nullptr, sm.flag, BDDA_SM_LU2);
+        const SmallState::Read rd = sm.read();
+        if (rd.flags & BDDA_SM_LU) return BDDA_EBREAKDOWN;
+        const double err = std::sqrt(std::max(rd.v, 0.0)) / bnorm;
         c.info.iters = k + 1;
         c.info.recurrence_res = err;
         if (c.hist) c.hist->push_back(err);
@@ -257,16 +326,13 @@ int run_bicgstab(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
             c.info.converged = 1;
             break;
         }
-        HostMat M3 = gram<R>(w, R0.p, rows, s, Z.p, rows, s, rows, 1);  // beta = LU(R0^H V)^{-1} (-R0^H Z)
-        HostMat beta;
-        if (!hm_lu_solve(M1, neg(M3), s, s, 0, beta)) return BDDA_EBREAKDOWN;
-        HostMat qb = scal(beta, mk(-qsi.x, -qsi.y));
+        if (rd.flags & BDDA_SM_LU2) return BDDA_EBREAKDOWN;
         {   // P = R + (P - qsi V) beta
             Combo<R> cb;
-            cb.nv = 1; cb.V[0] = Rb.p; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
+            cb.nv = 1; cb.V[0] = Rb.p; cb.ldv[0] = rows; cb.a[0] = one;
             cb.nm = 2;
-            cb.W[0] = pP; cb.ldw[0] = rows; cb.kw[0] = s; cb.M[0] = beta.data();
-            cb.W[1] = V.p; cb.ldw[1] = rows; cb.kw[1] = s; cb.M[1] = qb.data();
+            cb.W[0] = pP; cb.ldw[0] = rows; cb.kw[0] = s; cb.Md[0] = beta;
+            cb.W[1] = V.p; cb.ldw[1] = rows; cb.kw[1] = s; cb.Md[1] = beta; cb.ms[1] = mk(-1.0, 0.0); cb.msd[1] = qsi;
             combo<R>(w, cb, pPn, rows, rows, s);
         }
         std::swap(pP, pPn);
@@ -282,6 +348,7 @@ int run_bicggr(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     BlockWork& w = c.P.work;
     const int s = c.s;
     const int64_t rows = c.rows, N = rows * s;
+    const c64 one = mk(1.0, 0.0);
     DevBuf<cplx<R>> Rb(N), Rn(N), Pb(N), R0(N), V(N), W(N), U(N), Y(N);
     if (c.jacobi) scale_rows<R>(w.stream, c.P.jpre<R>(), B, rows, Rb.p, rows, rows, s);
     else copy_block<R>(w, B, Rb.p, rows, s);
@@ -292,41 +359,46 @@ int run_bicggr(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     ++c.info.matvecs;
     copy_block<R>(w, V.p, W.p, rows, s);
     copy_block<R>(w, Rb.p, R0.p, rows, s);
+    SmallState sm(w, s, 6, rows);
+    c64 *M1 = sm.m(0), *M2 = sm.m(1), *M3 = sm.m(2), *alpha = sm.m(3), *gamma = sm.m(4), *sc = sm.m(5);
+    c64 *wr = sc, *ww = sc + 1, *qsi = sc + 2;
     cplx<R>* pR = Rb.p;
     cplx<R>* pRn = Rn.p;
     for (int k = 0; k < itermax; ++k) {
-        HostMat M1 = gram<R>(w, R0.p, rows, s, V.p, rows, s, rows, 1);
-        HostMat M2 = gram<R>(w, R0.p, rows, s, pR, rows, s, rows, 1);
-        HostMat alpha;
-        if (!hm_lu_solve(M1, M2, s, s, 0, alpha)) return BDDA_EBREAKDOWN;
-        const c64 wr = dot_all<R>(w, W.p, rows, pR, rows, rows, s, 1);
-        const c64 ww = dot_all<R>(w, W.p, rows, W.p, rows, rows, s, 1);
-        const c64 qsi = cdiv(wr, ww);
-        HostMat qa = scal(alpha, mk(-qsi.x, -qsi.y));
+        gram_dev<R>(w, R0.p, rows, s, V.p, rows, s, rows, 1, M1);
+        gram_dev<R>(w, R0.p, rows, s, pR, rows, s, rows, 1, M2);
+        dm_lu_solve(w.stream, M1, M2, alpha, s, s, 0, one, nullptr, sm.flag, BDDA_SM_LU);
+        dot_all_dev<R>(w, W.p, rows, pR, rows, rows, s, 1, wr);
+        dot_all_dev<R>(w, W.p, rows, W.p, rows, rows, s, 1, ww);
+        dm_scalar_div(w.stream, wr, ww, one, qsi, sm.flag);
         {   // U = (P - qsi V) alpha
             Combo<R> cb;
             cb.nm = 2;
-            cb.W[0] = Pb.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.M[0] = alpha.data();
-            cb.W[1] = V.p; cb.ldw[1] = rows; cb.kw[1] = s; cb.M[1] = qa.data();
+            cb.W[0] = Pb.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.Md[0] = alpha;
+            cb.W[1] = V.p; cb.ldw[1] = rows; cb.kw[1] = s; cb.Md[1] = alpha; cb.ms[1] = mk(-1.0, 0.0); cb.msd[1] = qsi;
             combo<R>(w, cb, U.p, rows, rows, s);
         }
         problem_matvec<R>(c.P, U.p, Y.p, s, c.jacobi);  // Y = A U
         ++c.info.matvecs;
         {   // X = X + qsi R + U
             Combo<R> cb;
-            cb.nv = 3; cb.V[0] = X; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.V[1] = pR; cb.ldv[1] = rows; cb.a[1] = qsi;
-            cb.V[2] = U.p; cb.ldv[2] = rows; cb.a[2] = mk(1.0, 0.0);
+            cb.nv = 3; cb.V[0] = X; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.V[1] = pR; cb.ldv[1] = rows; cb.a[1] = one; cb.ad[1] = qsi;
+            cb.V[2] = U.p; cb.ldv[2] = rows; cb.a[2] = one;
             combo<R>(w, cb, X, rows, rows, s);
         }
         {   // Rnew = R - qsi W - Y
             Combo<R> cb;
-            cb.nv = 3; cb.V[0] = pR; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.V[1] = W.p; cb.ldv[1] = rows; cb.a[1] = mk(-qsi.x, -qsi.y);
+            cb.nv = 3; cb.V[0] = pR; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.V[1] = W.p; cb.ldv[1] = rows; cb.a[1] = mk(-1.0, 0.0); cb.ad[1] = qsi;
             cb.V[2] = Y.p; cb.ldv[2] = rows; cb.a[2] = mk(-1.0, 0.0);
             combo<R>(w, cb, pRn, rows, rows, s);
         }
-        const double err = norm_fro<R>(w, pRn, rows, rows, s) / bnorm;
+        dot_all_dev<R>(w, pRn, rows, pRn, rows, rows, s, 1, sm.probe);
+        const SmallState::Read rd = sm.read();
+        // (LU2: the previous iteration's gamma solve broke down)
+        if (rd.flags & BDDA_SM_LU_ANY) return BDDA_EBREAKDOWN;
+        const double err = std::sqrt(std::max(rd.v, 0.0)) / bnorm;
         c.info.iters = k + 1;
         c.info.recurrence_res = err;
         if (c.hist) c.hist->push_back(err);
@@ -337,26 +409,26 @@ int run_bicggr(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
         }
         problem_matvec<R>(c.P, pRn, W.p, s, c.jacobi);  // W = A Rnew
         ++c.info.matvecs;
-        HostMat M3 = gram<R>(w, R0.p, rows, s, pR, rows, s, rows, 1);   // R0^H R
-        HostMat M4 = gram<R>(w, R0.p, rows, s, pRn, rows, s, rows, 1);  // R0^H Rnew / qsi
-        const c64 iq = cdiv(mk(1.0, 0.0), qsi);
-        (void)iq;
-        for (auto& v : M4) v = cdiv(v, qsi);
-        HostMat gamma;
-        if (!hm_lu_solve(M3, M4, s, s, 0, gamma)) return BDDA_EBREAKDOWN;
+        gram_dev<R>(w, R0.p, rows, s, pR, rows, s, rows, 1, M3);   // R0^H R
+        gram_dev<R>(w, R0.p, rows, s, pRn, rows, s, rows, 1, M2);  // R0^H Rnew (divided by qsi in the solve)
+        dm_lu_solve(w.stream, M3, M2, gamma, s, s, 0, one, qsi, sm.flag, BDDA_SM_LU2);
         std::swap(pR, pRn);
         {   // P = R + U gamma
             Combo<R> cb;
-            cb.nv = 1; cb.V[0] = pR; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.nm = 1; cb.W[0] = U.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.M[0] = gamma.data();
+            cb.nv = 1; cb.V[0] = pR; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.nm = 1; cb.W[0] = U.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.Md[0] = gamma;
             combo<R>(w, cb, Pb.p, rows, rows, s);
         }
         {   // V = W + Y gamma
             Combo<R> cb;
-            cb.nv = 1; cb.V[0] = W.p; cb.ldv[0] = rows; cb.a[0] = mk(1.0, 0.0);
-            cb.nm = 1; cb.W[0] = Y.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.M[0] = gamma.data();
+            cb.nv = 1; cb.V[0] = W.p; cb.ldv[0] = rows; cb.a[0] = one;
+            cb.nm = 1; cb.W[0] = Y.p; cb.ldw[0] = rows; cb.kw[0] = s; cb.Md[0] = gamma;
             combo<R>(w, cb, V.p, rows, rows, s);
         }
+    }
+    if (!c.info.converged && c.info.iters > 0) {
+        // a gamma breakdown in the last iteration that ran
+        if (sm.read().flags & BDDA_SM_LU_ANY) return BDDA_EBREAKDOWN;
     }
     BDDA_HIP(hipStreamSynchronize(w.stream));
     return BDDA_OK;

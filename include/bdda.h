@@ -127,6 +127,16 @@ int bdda_residual_columns(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdd
  * coefficients, R(k,k) the norm after re-orthogonalisation). */
 int bdda_qr_reduced(const bdda_c64* A, int64_t m, int n, bdda_c64* Q, bdda_c64* R);
 
+/* X (s x nb) = A^{-1} B for the s x s matrices of the block solvers, on the
+ * GPU in FP64, by LU with partial (full = 0) or full (full = 1) pivoting:
+ * the Eigen PartialPivLU(A).solve(B) / FullPivLU(A).solve(B) calls of the
+ * solvers (bl_cocg_rq_jacobi_mvp_fft.cpp:63,73, bl_cocg_rq_mvp_fft.cpp:56,66,
+ * bl_bicgstab_jacobi_mvp_fft.cpp:62,93, bl_bicggr_jacobi_mvp_fft.cpp:58,113).
+ * The solvers run the same kernel on device-resident matrices.  Host
+ * buffers, column-major, 1 <= s <= 64, 0 <= nb <= 64.  An exactly zero pivot
+ * returns BDDA_EBREAKDOWN (Eigen would return non-finite values). */
+int bdda_lu_solve(const bdda_c64* A, const bdda_c64* B, bdda_c64* X, int s, int nb, int full);
+
 /* ---------------------------------------------------------------- observables
  * Cross sections of a solved block (Result_manager.cpp:65-122), on the GPU
  * in FP64.  B is the incident-field block the solve used (EHinc) and X the

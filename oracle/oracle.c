@@ -719,6 +719,11 @@ static void qr_reduced_z(const zc* A, int64_t m, int n, zc* Q, zc* R)
     }
 }
 
+void orc_lu_solve(const double* A, const double* B, int s, int nb, int full, double* X)
+{
+    lu_solve((const zc*)A, (const zc*)B, s, nb, full, (zc*)X);
+}
+
 int orc_qr_reduced(const double* A, int64_t m, int ncol, double* Q, double* R)
 {
     qr_reduced_z((const zc*)A, m, ncol, (zc*)Q, (zc*)R);

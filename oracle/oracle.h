@@ -66,6 +66,9 @@ int orc_block_matvec(const int* n, int f, const int* address, int64_t nocc,
                      const double* X, double* Y, int L, int nthreads);
 
 int orc_qr_reduced(const double* A, int64_t m, int ncol, double* Q, double* R);
+/* X (s x nb) = A^{-1} B, partial (full = 0) or full (full = 1) pivoting: the
+ * s x s solves of the block solvers (Eigen PartialPivLU / FullPivLU .solve()). */
+void orc_lu_solve(const double* A, const double* B, int s, int nb, int full, double* X);
 
 enum { ORC_COCG_RQ = 0, ORC_BICGSTAB = 1, ORC_BICGGR = 2 };
 /* Runs one of the six reference solvers.  hist (length itermax, may be NULL)

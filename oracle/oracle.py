@@ -49,6 +49,8 @@ def lib():
         L.orc_fft_config.argtypes = [ctypes.c_int64, ctypes.c_int]
         L.orc_fft_config.restype = None
         L.orc_qr_reduced.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, _dp]
+        L.orc_lu_solve.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_lu_solve.restype = None
         L.orc_bt_fft_custom.argtypes = [_ip, ctypes.c_int, ELEM_FN, ctypes.c_void_p, _dp]
         L.orc_block_matvec.argtypes = [_ip, ctypes.c_int, _ip, ctypes.c_int64, _dp, _dp,
                                        ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int]
@@ -167,6 +169,17 @@ def qr_reduced(A):
     R = np.zeros((nc, nc), dtype=np.complex128, order="F")
     lib().orc_qr_reduced(_d(A), m, nc, _d(Q), _d(R))
     return Q, R
+
+
+def lu_solve(A, B, full=False):
+    """X = A^{-1} B by LU with partial or full pivoting (the solvers' s x s solves)."""
+    A = np.asfortranarray(A, dtype=np.complex128)
+    B = np.asfortranarray(B, dtype=np.complex128)
+    s = A.shape[0]
+    B2 = B.reshape(s, -1, order="F")
+    X = np.zeros_like(B2, order="F")
+    lib().orc_lu_solve(_d(A), _d(B2), s, B2.shape[1], int(bool(full)), _d(X))
+    return X.reshape(B.shape, order="F")
 
 
 def block_matvec(n, f, address, Au_til, diag, X, jacobi=False, nthreads=1):

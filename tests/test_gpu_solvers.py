@@ -183,3 +183,28 @@ def test_solver_error_paths(bdda, orc):
     assert info["converged"] == 0 and info["status"] == 5 and info["iters"] == 1
     with pytest.raises(bdda.BddaError):
         op.solve(B, -1.0, 10)
+
+
+@pytest.mark.parametrize("full", [False, True])
+@pytest.mark.parametrize("s", [2, 16, 64])
+def test_lu_solve_gpu_matches_oracle(bdda, orc, s, full):
+    """The s x s LU solve the solvers run on the device (one work-group, FP64,
+    partial or full pivoting) against the oracle's lu_solve, to 1e-12, on a
+    random matrix and on one that needs pivoting; an exactly singular matrix
+    returns BDDA_EBREAKDOWN."""
+    print("case lu_solve s", s, "full", full, flush=True)
+    rng = np.random.default_rng(70 + s + full)
+    A = rng.standard_normal((s, s)) + 1j * rng.standard_normal((s, s))
+    B = rng.standard_normal((s, s)) + 1j * rng.standard_normal((s, s))
+    ref = orc.lu_solve(A, B, full)
+    X = bdda.lu_solve(A, B, full)
+    assert np.linalg.norm(X - ref) <= 1e-12 * np.linalg.norm(ref)
+    P = np.roll(np.eye(s), 1, axis=0) + 1e-3 * (rng.standard_normal((s, s)) + 1j * rng.standard_normal((s, s)))
+    P[0, 0] = 0.0
+    assert np.linalg.norm(bdda.lu_solve(P, B, full) - orc.lu_solve(P, B, full)) <= 1e-12 * np.linalg.norm(
+        orc.lu_solve(P, B, full))
+    S = A.copy()
+    S[:, 1] = 0.0
+    with pytest.raises(bdda.BddaError) as e:
+        bdda.lu_solve(S, B, full)
+    assert e.value.code == 4  # EBREAKDOWN

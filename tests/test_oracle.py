@@ -352,3 +352,24 @@ def test_qr_reduced_nearly_dependent_columns(orc):
     Qn, Rn = np.linalg.qr(A)
     ph = np.diag(Rn) / np.abs(np.diag(Rn))
     assert np.abs(Rn / ph[:, None] - R).max() <= 1e-12 * np.abs(R).max()
+
+
+@pytest.mark.parametrize("full", [False, True])
+@pytest.mark.parametrize("s", [1, 2, 7, 16, 64])
+def test_lu_solve_against_lapack(orc, s, full):
+    """The solvers' s x s solves (Eigen PartialPivLU / FullPivLU .solve(),
+    bl_cocg_rq_mvp_fft.cpp:56,66) against LAPACK's solve: a random complex
+    matrix, and one whose leading entries are zero so that the elimination
+    must pivot (rows, and for FullPivLU columns) to proceed."""
+    rng = np.random.default_rng(30 + s)
+    A = rng.standard_normal((s, s)) + 1j * rng.standard_normal((s, s))
+    B = rng.standard_normal((s, s)) + 1j * rng.standard_normal((s, s))
+    ref = np.linalg.solve(A, B)
+    X = orc.lu_solve(A, B, full)
+    assert np.linalg.norm(X - ref) <= 1e-12 * np.linalg.norm(ref) * max(1.0, np.linalg.cond(A) / 1e3)
+    if s >= 2:
+        P = np.roll(np.eye(s), 1, axis=0) + 1e-3 * (rng.standard_normal((s, s)) + 1j * rng.standard_normal((s, s)))
+        P[0, 0] = 0.0
+        X2 = orc.lu_solve(P, B, full)
+        ref2 = np.linalg.solve(P, B)
+        assert np.linalg.norm(X2 - ref2) <= 1e-12 * np.linalg.norm(ref2)
