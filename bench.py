@@ -338,7 +338,7 @@ SERVING_BUDGET = 240 << 30      # of the MI355X's 288 GB HBM
 SERVING_WS_LIMIT = 32 << 30     # block matvecs beyond this workspace run in column chunks
 
 
-def serving_leg(pkg, nreq, rate, nscenes=24, cap=SERVING_BUDGET):
+def serving_leg(pkg, nreq, rate, nscenes=24, cap=SERVING_BUDGET, workers=4):
     """Poisson stream of solve requests through the serving loop (mixed
     precision, block COCG-rq-Jacobi, tol 1e-5): p50/p99 latency, solves/s.
     The scenes follow the stream workload's distribution (N logUniform
@@ -366,7 +366,7 @@ def serving_leg(pkg, nreq, rate, nscenes=24, cap=SERVING_BUDGET):
         log("served %d/%d: batch of %d request(s), %d columns, %d dipoles, %.2f s" % (
             done[0], nreq, len(group), sum(r.cols for r in group), group[0].scene.target.nocc, secs))
 
-    sv = S.Server(be, hbm_budget=SERVING_BUDGET, workers=4, max_cols=64, on_batch=on_batch).start()
+    sv = S.Server(be, hbm_budget=SERVING_BUDGET, workers=workers, max_cols=64, on_batch=on_batch).start()
     t1 = time.time()
     S.replay(sv, reqs)
     sv.drain()
@@ -475,6 +475,8 @@ def main():
                     help="time the block matvecs and exit (for counter runs)")
     ap.add_argument("--serving-requests", type=int, default=200,
                     help="requests in the serving stream leg (0 skips it)")
+    ap.add_argument("--serving-workers", type=int, default=4,
+                    help="serving-loop workers (concurrent batches on the GPU)")
     ap.add_argument("--serving-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serving-cap-gib", type=int, default=48,
                     help="per-scene cap of the stream's scene draws (GiB a 64-column batch may need); 240 admits "
@@ -618,7 +620,8 @@ def main():
     L.bdda_dev_free(dB)
     L.bdda_dev_free(dX)
     op.close()
-    serving = (serving_leg(pkg, args.serving_requests, args.serving_rate, cap=args.serving_cap_gib << 30)
+    serving = (serving_leg(pkg, args.serving_requests, args.serving_rate, cap=args.serving_cap_gib << 30,
+                           workers=args.serving_workers)
                if args.serving_requests > 0 else None)
     sphere = None if args.no_sphere else sphere_leg(pkg)
     refpub = reference_leg(pkg, ref_sizes) if ref_sizes else None

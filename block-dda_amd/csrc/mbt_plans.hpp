@@ -12,22 +12,44 @@
 namespace bdda {
 
 // P, then its radices (forward order)
+// (every 2n - 1 up to 1023 finds a planned size at most 12.5 % above the
+// smallest 2/3/5/7-smooth one: sizes without a plan run the all-in-LDS
+// kernels, measured 3.4x slower at 280^3)
 #define BDDA_PLAN_TABLE(X) \
     X(32, 8, 4)            \
+    X(40, 8, 5)            \
     X(48, 8, 6)            \
+    X(56, 8, 7)            \
     X(64, 8, 8)            \
+    X(72, 8, 3, 3)         \
+    X(80, 8, 5, 2)         \
     X(96, 8, 4, 3)         \
+    X(112, 8, 7, 2)        \
+    X(120, 8, 5, 3)        \
     X(128, 8, 4, 4)        \
     X(160, 8, 5, 4)        \
+    X(168, 8, 7, 3)        \
     X(192, 8, 6, 4)        \
+    X(200, 8, 5, 5)        \
     X(224, 8, 7, 4)        \
+    X(240, 8, 6, 5)        \
     X(256, 8, 8, 4)        \
     X(280, 8, 7, 5)        \
+    X(288, 8, 6, 6)        \
     X(320, 8, 8, 5)        \
+    X(336, 8, 7, 6)        \
     X(343, 7, 7, 7)        \
     X(384, 8, 8, 6)        \
+    X(392, 8, 7, 7)        \
     X(448, 8, 8, 7)        \
-    X(512, 8, 8, 8)
+    X(512, 8, 8, 8)        \
+    X(576, 8, 8, 3, 3)     \
+    X(640, 8, 8, 5, 2)     \
+    X(720, 8, 6, 5, 3)     \
+    X(784, 8, 7, 7, 2)     \
+    X(896, 8, 8, 7, 2)     \
+    X(960, 8, 8, 5, 3)     \
+    X(1024, 8, 8, 8, 2)
 
 // calls f(Radices<...>{}) for the plan of size P; false when P has none
 template <class F>
@@ -616,13 +638,18 @@ inline bool launch_xfused_plan(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, 
         }
         // (odd line counts that fill the waves, 9 or 5 at 343, measured 19 % and
         // 14 % slower here: their row segments straddle 64-byte boundaries)
-        constexpr int NQ = T::NQ_X;
-        const int ntz = (int)ceil_div(d.Pz, NQ);
-        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * NQ);
-        constexpr int NT = plan_threads<PL, NQ, 2, 512>();
-        hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, NQ>), dim3((unsigned)((int64_t)s * d.Py * ((ntz + 1) & ~1))),
-                           dim3(NT), shm, st, V, gh, d, Pzs, tw, ntz, s);
-        BDDA_HIP(hipGetLastError());
+        if constexpr (F * PL::P > MBT_MAX_PF) {
+            fits = false;  // refused earlier by mbt_padded_dims; not instantiated
+        } else {
+            constexpr int NQ = T::NQ_X;
+            const int ntz = (int)ceil_div(d.Pz, NQ);
+            const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * NQ);
+            constexpr int NT = plan_threads<PL, NQ, 2, 512>();
+            hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, NQ>),
+                               dim3((unsigned)((int64_t)s * d.Py * ((ntz + 1) & ~1))), dim3(NT), shm, st, V, gh, d,
+                               Pzs, tw, ntz, s);
+            BDDA_HIP(hipGetLastError());
+        }
     });
     return planned && fits;
 }
@@ -645,18 +672,24 @@ inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx
 {
     using T = PlanTiles<R>;
     if (!zplan_ok<R, F>(d, Pzs)) return false;
-    return with_plan(d.Pz, [&](auto pl) {
+    bool fits = true;
+    const bool planned = with_plan(d.Pz, [&](auto pl) {
         using PL = decltype(pl);
-        constexpr int TL = T::TL_Z;  // more lines would cut the resident groups (LDS gather rows)
-        const int64_t nxny = (int64_t)d.nx * d.ny;
-        const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
-        if (ntiles == 0) return;
-        const size_t shm = sizeof(cplx<R>) * (PL::P + zpass_lds_elems<PL, F, TL, T::NCX_Z>());
-        constexpr int NT = plan_threads<PL, TL, 2, 256>();
-        hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
-                           X, ldx, c2o, U, d, Pzs, tw, ntiles, s, lines, nlines);
-        BDDA_HIP(hipGetLastError());
+        if constexpr (F * PL::P > MBT_MAX_PF) {
+            fits = false;  // refused earlier by mbt_padded_dims; not instantiated
+        } else {
+            constexpr int TL = T::TL_Z;  // more lines would cut the resident groups (LDS gather rows)
+            const int64_t nxny = (int64_t)d.nx * d.ny;
+            const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
+            if (ntiles == 0) return;
+            const size_t shm = sizeof(cplx<R>) * (PL::P + zpass_lds_elems<PL, F, TL, T::NCX_Z>());
+            constexpr int NT = plan_threads<PL, TL, 2, 256>();
+            hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm,
+                               st, X, ldx, c2o, U, d, Pzs, tw, ntiles, s, lines, nlines);
+            BDDA_HIP(hipGetLastError());
+        }
     });
+    return planned && fits;
 }
 
 template <typename R, int F>
@@ -667,18 +700,24 @@ inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X,
 {
     using T = PlanTiles<R>;
     if (!zplan_ok<R, F>(d, Pzs)) return false;
-    return with_plan(d.Pz, [&](auto pl) {
+    bool fits = true;
+    const bool planned = with_plan(d.Pz, [&](auto pl) {
         using PL = decltype(pl);
-        constexpr int TL = plan_lines<PL, T::TL_Z, 2, 256>();
-        const int64_t nxny = (int64_t)d.nx * d.ny;
-        const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
-        if (ntiles == 0) return;
-        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)T::NCX_Z * TL * PL::P);
-        constexpr int NT = plan_threads<PL, TL, 2, 256>();
-        hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
-                           U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles, s, lines, nlines);
-        BDDA_HIP(hipGetLastError());
+        if constexpr (F * PL::P > MBT_MAX_PF) {
+            fits = false;  // refused earlier by mbt_padded_dims; not instantiated
+        } else {
+            constexpr int TL = plan_lines<PL, T::TL_Z, 2, 256>();
+            const int64_t nxny = (int64_t)d.nx * d.ny;
+            const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
+            if (ntiles == 0) return;
+            const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)T::NCX_Z * TL * PL::P);
+            constexpr int NT = plan_threads<PL, TL, 2, 256>();
+            hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm,
+                               st, U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles, s, lines, nlines);
+            BDDA_HIP(hipGetLastError());
+        }
     });
+    return planned && fits;
 }
 
 }  // namespace bdda

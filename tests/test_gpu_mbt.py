@@ -171,12 +171,43 @@ def test_x_plan_192(bdda, orc, n):
         assert rel(op.apply_occupied(X, "fp32", jac), ref) <= TOL32
 
 
+# padded sizes with a compile-time plan: every size of the table below 512
+# and the four-stage ones above, each landed on one axis (extent P / 2)
+PLAN_SIZES = [40, 56, 72, 80, 112, 120, 168, 200, 240, 288, 336, 392, 576, 640, 720, 784, 896, 960, 1024]
+PLAN_CASES = [(P, ax, 3) for P in PLAN_SIZES for ax in (0, 1, 2)]
+PLAN_CASES += [(P, ax, 6) for P in (40, 56, 72, 80, 112, 120, 168, 200, 240, 288, 336, 392) for ax in (0, 2)]
+
+
+@pytest.mark.parametrize("P,axis,f", PLAN_CASES)
+def test_plan_sizes(bdda, orc, P, axis, f):
+    """Occupied matvec on a grid one of whose axes pads to the plan size P
+    (the others stay small and run the all-in-LDS kernels) against the
+    oracle, FP64 and FP32, with Jacobi scaling, over 2 right-hand sides."""
+    print("case plan size", P, "axis", axis, "f", f, flush=True)
+    rng = np.random.default_rng(P * 7 + axis * 3 + f)
+    n = [5, 4, 6]
+    n[axis] = P // 2
+    n = tuple(n)
+    C = int(np.prod(n))
+    addr = np.sort(rng.choice(C, size=C // 3, replace=False)).astype(np.int32)
+    diag = np.repeat(1.0 / (rng.standard_normal(addr.size) * 0.1 + 0.3 + 0.05j), f)
+    lf, k = 0.21, 1.4 + 0.01j
+    op = bdda.MBT_fft_init(n, f, lf, k)
+    assert op.grid[axis] == P
+    op.set_target(addr, diag)
+    Au = orc.mbt_fft_init(n, f, lf, k)
+    X = rnd(rng, f * addr.size, 2)
+    ref = orc.block_matvec(n, f, addr, Au, diag, X, jacobi=True)
+    assert rel(op.apply_occupied(X, "fp64", True), ref) <= TOL64
+    assert rel(op.apply_occupied(X, "fp32", True), ref) <= TOL32
+
+
 @pytest.mark.parametrize("jac", [False, True])
 def test_occupied_lines_against_cuboid(bdda, jac):
     """Occupied matvec (only occupied lines transformed along z) against the
     cuboid-layout matvec (every line transformed) of the same operator, on
-    100x128x120 -> 224x256x256, where the y passes hold two butterflies per
-    thread in their first and last stages.  30 % of the (ix, iy) lines are
+    100x128x120 (y padded to 256, where the y passes hold two butterflies per
+    thread in their first and last stages).  30 % of the (ix, iy) lines are
     occupied, each by a random subset of its cells."""
     print("case occupied lines vs cuboid jacobi", jac, flush=True)
     rng = np.random.default_rng(41)
@@ -189,7 +220,7 @@ def test_occupied_lines_against_cuboid(bdda, jac):
     m = addr.size
     diag = np.repeat(1.0 / (rng.standard_normal(m) * 0.1 + 0.3 + 0.05j), f)
     op = bdda.MBT_fft_init(n, f, 0.02, 2.5)
-    assert op.grid == (224, 256, 256)
+    assert op.grid[1] == 256
     op.set_target(addr, diag)
     X = rnd(rng, f * m, 2)
     C = int(np.prod(n))
