@@ -544,6 +544,19 @@ def main():
         log("matvec %s: %s  total %.2f ms (%.0f GB/s algorithmic)" % (
             name, {kk: round(v, 3) for kk, v in ms.items()}, sum(ms.values()), mv[name]["matvec_GBs"]))
 
+    # what a plain streaming kernel reaches on this GPU (2 x 2 GiB buffers)
+    probe = None
+    pa, pb = ctypes.c_void_p(), ctypes.c_void_p()
+    pbytes = 2 << 30
+    if L.bdda_dev_alloc(pbytes, ctypes.byref(pa)) == 0:
+        if L.bdda_dev_alloc(pbytes, ctypes.byref(pb)) == 0:
+            cg, rg, wg = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            pkg._check(L.bdda_hbm_probe(pa, pb, pbytes, ctypes.byref(cg), ctypes.byref(rg), ctypes.byref(wg)))
+            probe = {"copy_GBs": cg.value, "read_GBs": rg.value, "write_GBs": wg.value, "buffer_bytes": pbytes,
+                     "kernel": "bdda_hbm_probe: 16-byte vector streaming copy/read/write, best of 5"}
+            L.bdda_dev_free(pb)
+        L.bdda_dev_free(pa)
+    log("hbm probe: %s" % probe)
     if args.copy_bytes > 0:
         cA, cB = ctypes.c_void_p(), ctypes.c_void_p()
         pkg._check(L.bdda_dev_alloc(args.copy_bytes, ctypes.byref(cA)))
@@ -588,6 +601,13 @@ def main():
     pkg._check(L.bdda_device_sync())
     total = time.perf_counter() - t_all
     runs[-1]["true_res_col_max"] = residual_col_max(pkg, op, dB, dX, s)
+    # the same solve stopping on every column's true residual (no reference
+    # counterpart: the reference's rule is the block norm)
+    cols = solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_MIXED_COLUMNS, method)
+    cols["true_res_col_max"] = residual_col_max(pkg, op, dB, dX, s)
+    log("mixed, per-column stop: %s" % cols)
+    # X of the block-rule solve again for the observables below
+    solve_dev(pkg, op, dB, dX, s, args.tol, args.itermax, pkg.MODE_MIXED, method)
     obs = None
     if obs64 is not None:
         aeff = (3.0 * tgt.nocc * tgt.lf ** 3 / (4.0 * math.pi)) ** (1.0 / 3.0)
@@ -607,6 +627,7 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS,
                 "traffic": float(counters["x_traffic_bytes"]) if counters else None,
                 "traffic_source": counters_note,
+                "frac_of_streaming_copy": (achieved / probe["copy_GBs"]) if probe else None,
                 "kernel": "k_xfused_plan<float> at Px=%d (x FFT fwd + G^ multiply + x FFT inv)" % P[0],
                 "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms}
     if counters:
@@ -646,6 +667,8 @@ def main():
                   "true_res": runs[-1]["true_res"], "true_res_col_max": runs[-1]["true_res_col_max"],
                   "converged": runs[-1]["converged"], "wall_s_median": mixed_median,
                   "wall_s_runs": [r["wall_s"] for r in runs]},
+        "mixed_per_column_stop": {k2: cols[k2] for k2 in ("iters", "outer_iters", "matvecs", "true_res",
+                                                           "true_res_col_max", "converged", "wall_s")},
         "fp64_time_s": fp64["wall_s_median"] if fp64 else None,
         "fp64": {k2: fp64[k2] for k2 in ("iters", "matvecs", "true_res", "true_res_col_max", "recurrence_res",
                                           "converged", "wall_s_median", "wall_s_runs")}
@@ -655,6 +678,7 @@ def main():
         "observables": obs,
         "serving": serving,
         "init_s": init_s,
+        "hbm_probe": probe,
         "matvec": {kk: {"pass_ms": v["pass_ms"], "matvec_GBs": v["matvec_GBs"],
                         "pass_GBs": v["pass_GBs"]} for kk, v in mv.items()},
         "roofline": roofline,

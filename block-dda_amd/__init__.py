@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(HERE, "libbdda.so")
 
 BDDA_PREC_FP64 = 1
 BDDA_PREC_FP32 = 2
-MODE_FP64, MODE_FP32, MODE_MIXED = 0, 1, 2
+MODE_FP64, MODE_FP32, MODE_MIXED, MODE_MIXED_COLUMNS = 0, 1, 2, 3
 COCG_RQ, BICGSTAB, BICGGR = 0, 1, 2
 STATUS = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "EBREAKDOWN", 5: "ENOCONV", 6: "ENAN",
           7: "EUNSUPPORTED"}
@@ -82,6 +82,7 @@ SIGNATURES = {
                                   ctypes.c_int]),
     "bdda_qr_reduced": (ctypes.c_int, [_vp, _i64, ctypes.c_int, _vp, _vp]),
     "bdda_lu_solve": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "bdda_hbm_probe": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _dp, _dp, _dp]),
     "bdda_dev_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "bdda_dev_free": (ctypes.c_int, [_vp]),
     "bdda_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),

@@ -588,6 +588,74 @@ __global__ void k_zero(cplx<R>* Z, int64_t ldz, int64_t rows)
         Z[j * ldz + r] = czero<R>();
 }
 
+// ------------------------------------------------------------- HBM probe
+// What this GPU's HBM delivers to a plain streaming kernel: 16-byte vector
+// accesses, four in flight per thread, grid-stride over the buffer.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_copy(const v4f* __restrict__ a, v4f* __restrict__ b, size_t n)
+{
+    const size_t st = (size_t)gridDim.x * 256;
+    size_t i = blockIdx.x * (size_t)256 + threadIdx.x;
+    for (; i + 3 * st < n; i += 4 * st) {
+        const v4f v0 = __builtin_nontemporal_load(a + i), v1 = __builtin_nontemporal_load(a + i + st);
+        const v4f v2 = __builtin_nontemporal_load(a + i + 2 * st), v3 = __builtin_nontemporal_load(a + i + 3 * st);
+        __builtin_nontemporal_store(v0, b + i);
+        __builtin_nontemporal_store(v1, b + i + st);
+        __builtin_nontemporal_store(v2, b + i + 2 * st);
+        __builtin_nontemporal_store(v3, b + i + 3 * st);
+    }
+    for (; i < n; i += st) b[i] = a[i];
+}
+__global__ __launch_bounds__(256) void k_stream_read(const v4f* __restrict__ a, size_t n, float* __restrict__ sink)
+{
+    const size_t st = (size_t)gridDim.x * 256;
+    float acc = 0.f;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += st) {
+        const v4f v = a[i];
+        acc += v.x + v.y + v.z + v.w;
+    }
+    if (acc == -1.2345e-31f) sink[0] = acc;  // keeps the loads; never true for the probe's zeros
+}
+__global__ __launch_bounds__(256) void k_stream_write(v4f* __restrict__ b, size_t n)
+{
+    const size_t st = (size_t)gridDim.x * 256;
+    const v4f z = {0.f, 0.f, 0.f, 0.f};
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += st) b[i] = z;
+}
+
+void hbm_probe(void* a, void* b, size_t bytes, double* copy_gbs, double* read_gbs, double* write_gbs)
+{
+    const size_t n = bytes / 16;
+    const dim3 grid(16384), block(256);
+    hipEvent_t e0, e1;
+    BDDA_HIP(hipEventCreate(&e0));
+    BDDA_HIP(hipEventCreate(&e1));
+    struct Ev {
+        hipEvent_t a, b;
+        ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } ev{e0, e1};
+    hipLaunchKernelGGL(k_stream_write, grid, block, 0, nullptr, (v4f*)a, n);  // defined contents
+    BDDA_HIP(hipGetLastError());
+    double best[3] = {0, 0, 0};
+    for (int rep = 0; rep < 6; ++rep)
+        for (int kind = 0; kind < 3; ++kind) {
+            BDDA_HIP(hipEventRecord(e0, nullptr));
+            if (kind == 0) hipLaunchKernelGGL(k_stream_copy, grid, block, 0, nullptr, (const v4f*)a, (v4f*)b, n);
+            if (kind == 1) hipLaunchKernelGGL(k_stream_read, grid, block, 0, nullptr, (const v4f*)a, n, (float*)b);
+            if (kind == 2) hipLaunchKernelGGL(k_stream_write, grid, block, 0, nullptr, (v4f*)b, n);
+            BDDA_HIP(hipGetLastError());
+            BDDA_HIP(hipEventRecord(e1, nullptr));
+            BDDA_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            BDDA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const double moved = (kind == 0 ? 2.0 : 1.0) * (double)(n * 16);
+            if (rep > 0) best[kind] = std::max(best[kind], moved / (ms * 1e-3) / 1e9);
+        }
+    *copy_gbs = best[0];
+    *read_gbs = best[1];
+    *write_gbs = best[2];
+}
+
 // ------------------------------------------------------------- host
 // rows x s element kernels: blockIdx.y is the column
 static dim3 grid_cols(int64_t rows, int s)

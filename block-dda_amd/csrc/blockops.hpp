@@ -142,6 +142,10 @@ template <typename R>
 void qr_fast_dev(BlockWork& w, const cplx<R>* A, int64_t lda, cplx<R>* Q, int64_t ldq, int64_t rows, int s, c64* Rd,
                  cplx<R>* scratch, c64* tmp, int* flag);
 
+// streaming copy, read and write rates (GB/s, best of five) over two device
+// buffers of `bytes` each: the HBM bandwidth a plain kernel reaches
+void hbm_probe(void* a, void* b, size_t bytes, double* copy_gbs, double* read_gbs, double* write_gbs);
+
 // ---------------- small dense FP64 algebra on the host (s <= 64)
 HostMat hm_mul(const HostMat& A, const HostMat& B, int m, int k, int n);       // A (m x k) B (k x n)
 HostMat hm_tmul(const HostMat& A, const HostMat& B, int k, int m, int n);      // A^T (A: k x m) B (k x n)

@@ -153,7 +153,7 @@ static int solve_common(bdda_mbt* h, int method, int jacobi, int mode, const c64
                         c64* dX, bdda_solve_info* info, double* hist, int hist_len)
 {
     if (method < 0 || method > 2) return fail(BDDA_EINVAL, "unknown method");
-    if (mode < 0 || mode > 2) return fail(BDDA_EINVAL, "unknown mode");
+    if (mode < 0 || mode > 3) return fail(BDDA_EINVAL, "unknown mode");
     if (L < 1 || L > 64) return fail(BDDA_EINVAL, "number of right-hand sides must be in 1..64");
     if (!(tol > 0) || itermax < 1) return fail(BDDA_EINVAL, "tol must be > 0 and itermax >= 1");
     if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
@@ -163,7 +163,8 @@ static int solve_common(bdda_mbt* h, int method, int jacobi, int mode, const c64
     SolveInfo si;
     std::vector<double> hv;
     int rc = solve_fp(h->prob, method, jacobi, mode, dB, dX, L, tol, itermax, si, hist ? &hv : nullptr, 0.0);
-    if (rc == BDDA_OK && mode != MODE_MIXED) si.true_res = true_residual(h->prob, dB, dX, L);
+    if (rc == BDDA_OK && mode != MODE_MIXED && mode != MODE_MIXED_COLUMNS)
+        si.true_res = true_residual(h->prob, dB, dX, L);
     if (rc == BDDA_OK && !si.converged) rc = BDDA_ENOCONV;
     si.status = rc;
     if (info) {
@@ -540,7 +541,8 @@ int bdda_estimate_bytes(const int n[3], int f, int64_t nocc, int L, int mode, in
     if (!n || !init_peak || !resident || !workspace || !krylov) return fail(BDDA_EINVAL, "null pointer");
     if (f != 3 && f != 6) return fail(BDDA_EINVAL, "unknown f value (3 or 6)");
     if (n[0] < 1 || n[1] < 1 || n[2] < 1 || nocc < 1 || L < 1) return fail(BDDA_EINVAL, "bad sizes");
-    if (mode != BDDA_MODE_FP64 && mode != BDDA_MODE_FP32 && mode != BDDA_MODE_MIXED) return fail(BDDA_EINVAL, "bad mode");
+    if (mode < BDDA_MODE_FP64 || mode > BDDA_MODE_MIXED_COLUMNS) return fail(BDDA_EINVAL, "bad mode");
+    if (mode == BDDA_MODE_MIXED_COLUMNS) mode = BDDA_MODE_MIXED;  // the same buffers
     int P[3];
     if (!mbt_padded_dims(n, f, P))
         return fail(BDDA_EUNSUPPORTED, "padded grid beyond the passes' limit (P * f <= 3072 on every axis)");
@@ -627,6 +629,16 @@ int bdda_lu_solve(const bdda_c64* A, const bdda_c64* B, bdda_c64* X, int s, int 
         BDDA_HIP(hipStreamSynchronize(st));
         if (f) return fail(BDDA_EBREAKDOWN, "lu_solve: exactly zero pivot");
         if (nx) BDDA_HIP(hipMemcpy(X, dX, sizeof(c64) * nx, hipMemcpyDeviceToHost));
+        return BDDA_OK;
+    })
+}
+
+int bdda_hbm_probe(void* a, void* b, size_t bytes, double* copy_gbs, double* read_gbs, double* write_gbs)
+{
+    if (!a || !b || !copy_gbs || !read_gbs || !write_gbs) return fail(BDDA_EINVAL, "null pointer");
+    if (bytes < (1u << 20)) return fail(BDDA_EINVAL, "use buffers of at least 1 MiB");
+    BDDA_TRY({
+        hbm_probe(a, b, bytes, copy_gbs, read_gbs, write_gbs);
         return BDDA_OK;
     })
 }

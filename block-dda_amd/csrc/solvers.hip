@@ -503,7 +503,7 @@ int solve_fp(Problem& P, int method, int jacobi, int mode, const c64* B64, c64* 
         fill_zero<double>(w.stream, X64, rows, rows, s);
         accumulate_32_to_64(w.stream, X32.p, rows, X64, rows, rows, s, 1.0);
         info.outer_iters = 1;
-    } else if (mode == MODE_MIXED) {
+    } else if (mode == MODE_MIXED || mode == MODE_MIXED_COLUMNS) {
         // FP64 defect correction: r = B - A x (FP64 matvec), A d = r in FP32,
         // x += d, until ||B - A x|| / ||B|| <= tol.
         DevBuf<c64> R64(N), AX(N);
@@ -517,9 +517,24 @@ int solve_fp(Problem& P, int method, int jacobi, int mode, const c64* B64, c64* 
         // every pass measures the residual of the current x first, so the
         // reported true residual and convergence always describe the X
         // returned, also after the last correction
+        // per-column test: ||b_j|| once, ||r_j|| from the diagonal of R^H R
+        std::vector<double> bcol;
+        if (mode == MODE_MIXED_COLUMNS) {
+            const HostMat gb = gram<double>(w, B64, rows, s, B64, rows, s, rows, 1);
+            for (int j = 0; j < s; ++j) bcol.push_back(std::sqrt(std::max(gb[(size_t)j * s + j].x, 0.0)));
+        }
         for (int o = 0;; ++o) {
             const double rn = norm_fro<double>(w, R64.p, rows, rows, s);
             rel = rn / bn;
+            if (mode == MODE_MIXED_COLUMNS) {
+                const HostMat gr = gram<double>(w, R64.p, rows, s, R64.p, rows, s, rows, 1);
+                double worst = 0;
+                for (int j = 0; j < s; ++j) {
+                    const double r = std::sqrt(std::max(gr[(size_t)j * s + j].x, 0.0));
+                    worst = std::max(worst, bcol[j] > 0 ? r / bcol[j] : r);
+                }
+                rel = worst;
+            }
             info.true_res = rel;
             if (hist) hist->push_back(-rel);  // negative marks an outer (true) residual
             if (rel <= tol) {

@@ -7,7 +7,7 @@
 namespace bdda {
 
 enum Method : int { M_COCG_RQ = 0, M_BICGSTAB = 1, M_BICGGR = 2 };
-enum Mode : int { MODE_FP64 = 0, MODE_FP32 = 1, MODE_MIXED = 2 };
+enum Mode : int { MODE_FP64 = 0, MODE_FP32 = 1, MODE_MIXED = 2, MODE_MIXED_COLUMNS = 3 };
 
 struct SolveInfo {
     int iters = 0;          // inner (block) iterations executed, summed over outer steps

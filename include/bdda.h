@@ -44,7 +44,7 @@ enum {
 
 /* precision masks / modes */
 enum { BDDA_PREC_FP64 = 1, BDDA_PREC_FP32 = 2 };
-enum { BDDA_MODE_FP64 = 0, BDDA_MODE_FP32 = 1, BDDA_MODE_MIXED = 2 };
+enum { BDDA_MODE_FP64 = 0, BDDA_MODE_FP32 = 1, BDDA_MODE_MIXED = 2, BDDA_MODE_MIXED_COLUMNS = 3 };
 enum { BDDA_COCG_RQ = 0, BDDA_BICGSTAB = 1, BDDA_BICGGR = 2 };
 
 typedef struct {
@@ -107,7 +107,11 @@ int bdda_mbt_apply_occupied(bdda_mbt* h, int prec, int jacobi, const bdda_c64* X
  * mode BDDA_MODE_FP64 keeps the reference's FP64 arithmetic and stopping
  * rule (recurrence residual < tol); BDDA_MODE_MIXED runs the inner block
  * iteration in complex FP32 inside an FP64 defect-correction loop that stops
- * on the true FP64 residual ||B - A X||/||B|| <= tol.  hist (optional,
+ * on the true FP64 residual ||B - A X||/||B|| <= tol; BDDA_MODE_MIXED_COLUMNS
+ * (no reference counterpart: the reference stops on the block norm, so one
+ * column may stay above tol) is the same loop stopping only when every
+ * column's true residual ||B_j - A X_j||/||B_j|| <= tol, and info.true_res
+ * is then the largest of them.  hist (optional,
  * hist_len entries) receives per-iteration residuals (outer true residuals
  * are stored negated; entries past the last one written are NaN).  Returns
  * BDDA_ENOCONV when itermax was reached. */
@@ -189,6 +193,13 @@ int bdda_mbt_set_workspace_limit(bdda_mbt* h, size_t bytes);
 /* ---------------------------------------------------------------- device
  * Device-resident variants (pointers from bdda_dev_alloc, complex<double>
  * layout) for callers that keep B and X in HBM. */
+/* HBM bandwidth that a plain streaming kernel of this library reaches (no
+ * reference counterpart; measurement only): best of five launches each of a
+ * copy a -> b (16-byte non-temporal vector accesses, rate counts read plus
+ * write bytes), a read of a and a write of b.  a and b are device buffers of
+ * `bytes` each (from bdda_dev_alloc); their contents are overwritten. */
+int bdda_hbm_probe(void* a, void* b, size_t bytes, double* copy_gbs, double* read_gbs, double* write_gbs);
+
 int bdda_dev_alloc(size_t bytes, void** p);
 int bdda_dev_free(void* p);
 int bdda_memcpy_h2d(void* dst, const void* src, size_t bytes);

@@ -6,7 +6,8 @@
     sections agree within 1e-4;
   * the wavelength sweep on the 4096-monomer aggregate (6 dipoles across, m
     linear in wavelength, s = 8) at its two ends and middle, solved with
-    mixed-precision block BiCGGR-Jacobi against FP64 BiCGGR-Jacobi.
+    mixed-precision block BiCGGR-Jacobi against FP64 BiCGGR-Jacobi, and at
+    all 64 of its wavelengths in mixed precision.
 """
 import importlib
 
@@ -50,3 +51,29 @@ def test_wavelength_sweep_bicggr(bdda):
     for lam in (300.0, 650.0, 1000.0):
         W = first if lam == 300.0 else wl.sweep(lam, target=first.target)
         _check_pair(bdda, W, bdda.BICGGR)
+
+
+def test_wavelength_sweep_all_64(bdda):
+    """All 64 wavelengths of the sweep (uniform in 300-1000 nm, m(lambda)
+    linear from 2.0 + 1.0i to 2.4 + 0.5i, s = 8), each with its own operator
+    and DIAG_A, solved by mixed-precision block BiCGGR-Jacobi to a true
+    residual of 1e-5; the cross sections are those of an absorbing particle
+    (Cext > Cabs > 0) and vary smoothly with the wavelength."""
+    wl = importlib.import_module("block_dda_amd.workloads")
+    first = wl.sweep(300.0)
+    t = first.target
+    print("case sweep 64", first.describe(), flush=True)
+    qext = []
+    for lam in np.linspace(300.0, 1000.0, 64):
+        W = first if lam == 300.0 else wl.sweep(lam, target=t)
+        op = bdda.MBT_fft_init(t.n, 3, t.lf, W.k)
+        op.set_target(t.address, W.diag)
+        X, info, _ = op.solve(W.B, 1e-5, 3000, method=bdda.BICGGR, jacobi=True, mode=bdda.MODE_MIXED)
+        assert info["converged"] and info["true_res"] <= 1e-5, (lam, info)
+        xs = op.cross_sections(W.B, X)
+        assert np.all(xs["Cabs"] > 0) and np.all(xs["Cext"] > xs["Cabs"]), lam
+        qext.append(float(np.mean(xs["Cext"])))
+        op.close()
+    q = np.array(qext)
+    # neighbouring wavelengths (11 nm apart) differ by a few per cent at most
+    assert np.max(np.abs(np.diff(q)) / q[1:]) < 0.1

@@ -92,3 +92,32 @@ def test_sphere16_mixed_vs_fp64_oracle_and_mie(bdda, orc, T):
     aeff = (3 * t.nocc * t.lf ** 3 / (4 * np.pi)) ** (1 / 3)
     M = mie.mie(W.cfg["m"], k * aeff)
     assert np.all(np.abs(got["Cext"] / (np.pi * aeff ** 2) - M["Qext"]) <= 0.05 * M["Qext"])
+
+
+def test_sphere16_fp64_cocg_rq_tol_1e6(bdda, orc, T):
+    """The sphere workload as stated: 16 dipoles across (2,106 dipoles, grid
+    16^3 padded to 32^3), m = 1.33, ka = 1, LDR, s = 2 (x/y-polarised waves
+    along +z), plain block COCG-rq (FullPivLU, no Jacobi) in complex FP64 to
+    a relative residual of 1e-6.  Its cross sections equal the oracle's FP64
+    CPU solve of the same system to 1e-6 and Mie's Qext within the DDA
+    discretisation error (5 %); Qabs is zero for the lossless sphere."""
+    mie = importlib.import_module("block_dda_amd.mie")
+    W = importlib.import_module("block_dda_amd.workloads").build("sphere16")
+    t, k = W.target, W.k
+    print("case sphere16 fp64 cocg_rq 1e-6", t.n, t.nocc, flush=True)
+    op = bdda.MBT_fft_init(t.n, 3, t.lf, k)
+    op.set_target(t.address, W.diag)
+    assert op.grid == (32, 32, 32) and t.nocc == 2106
+    X, info, _ = op.solve(W.B, 1e-6, 500, method=bdda.COCG_RQ, jacobi=False, mode=bdda.MODE_FP64)
+    assert info["converged"] and info["recurrence_res"] < 1e-6 and info["true_res"] <= 2e-6
+    Au = orc.mbt_fft_init(t.n, 3, t.lf, k)
+    Xo, _, _ = orc.solve("cocg_rq", t.n, 3, t.address, Au, W.diag, W.B, 1e-10, 500, jacobi=False, nthreads=4)
+    ref = orc.cross_sections(W.B, Xo, W.diag, 3, k)
+    got = op.cross_sections(W.B, X)
+    for q, r in zip(("Cext", "Cabs", "Csca"), ref):
+        assert np.abs(got[q] - r).max() <= 1e-5 * np.abs(ref[0]).max(), q
+    aeff = (3 * t.nocc * t.lf ** 3 / (4 * np.pi)) ** (1 / 3)
+    M = mie.mie(W.cfg["m"], k * aeff)
+    Q = got["Cext"] / (np.pi * aeff ** 2)
+    assert np.all(np.abs(Q - M["Qext"]) <= 0.05 * M["Qext"])
+    assert np.all(np.abs(got["Cabs"]) <= 1e-6 * np.abs(got["Cext"]).max())
