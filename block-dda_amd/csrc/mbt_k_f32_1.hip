@@ -1,0 +1,13 @@
+// mbt_k_f32_1.hip -- FP32 kernels of the planned passes for group 1 of the plan table
+// (explicit instantiations of the per-plan launch functions of mbt_plans.hpp;
+// the table's groups compile as separate units, in parallel).
+#define BDDA_PLAN_INSTANTIATE
+#include "mbt_plans.hpp"
+
+namespace bdda {
+#define BDDA_PLAN_DEF(p, ...) BDDA_PLAN_INST(, float, PlanOf##p)
+BDDA_PLAN_TABLE_1(BDDA_PLAN_DEF)
+#undef BDDA_PLAN_DEF
+BDDA_PLAN_INST_X(, float, PlanX192F32, 3)
+BDDA_PLAN_INST_X(, float, PlanX192F32, 6)
+}  // namespace bdda

@@ -15,41 +15,46 @@ namespace bdda {
 // (every 2n - 1 up to 1023 finds a planned size at most 12.5 % above the
 // smallest 2/3/5/7-smooth one: sizes without a plan run the all-in-LDS
 // kernels, measured 3.4x slower at 280^3)
-#define BDDA_PLAN_TABLE(X) \
-    X(32, 8, 4)            \
-    X(40, 8, 5)            \
-    X(48, 8, 6)            \
-    X(56, 8, 7)            \
-    X(64, 8, 8)            \
-    X(72, 8, 3, 3)         \
-    X(80, 8, 5, 2)         \
-    X(96, 8, 4, 3)         \
-    X(112, 8, 7, 2)        \
-    X(120, 8, 5, 3)        \
-    X(128, 8, 4, 4)        \
-    X(160, 8, 5, 4)        \
-    X(168, 8, 7, 3)        \
-    X(192, 8, 6, 4)        \
-    X(200, 8, 5, 5)        \
-    X(224, 8, 7, 4)        \
-    X(240, 8, 6, 5)        \
-    X(256, 8, 8, 4)        \
-    X(280, 8, 7, 5)        \
-    X(288, 8, 6, 6)        \
-    X(320, 8, 8, 5)        \
-    X(336, 8, 7, 6)        \
-    X(343, 7, 7, 7)        \
-    X(384, 8, 8, 6)        \
-    X(392, 8, 7, 7)        \
-    X(448, 8, 8, 7)        \
-    X(512, 8, 8, 8)        \
-    X(576, 8, 8, 3, 3)     \
-    X(640, 8, 8, 5, 2)     \
-    X(720, 8, 6, 5, 3)     \
-    X(784, 8, 7, 7, 2)     \
-    X(896, 8, 8, 7, 2)     \
-    X(960, 8, 8, 5, 3)     \
+#define BDDA_PLAN_TABLE_0(X) \
+    X(32, 8, 4)              \
+    X(40, 8, 5)              \
+    X(48, 8, 6)              \
+    X(56, 8, 7)              \
+    X(64, 8, 8)              \
+    X(72, 8, 3, 3)           \
+    X(80, 8, 5, 2)           \
+    X(96, 8, 4, 3)           \
+    X(112, 8, 7, 2)          \
+    X(120, 8, 5, 3)          \
+    X(128, 8, 4, 4)          \
+    X(160, 8, 5, 4)
+#define BDDA_PLAN_TABLE_1(X) \
+    X(168, 8, 7, 3)          \
+    X(192, 8, 6, 4)          \
+    X(200, 8, 5, 5)          \
+    X(224, 8, 7, 4)          \
+    X(240, 8, 6, 5)          \
+    X(256, 8, 8, 4)          \
+    X(280, 8, 7, 5)          \
+    X(288, 8, 6, 6)          \
+    X(320, 8, 8, 5)
+#define BDDA_PLAN_TABLE_2(X) \
+    X(336, 8, 7, 6)          \
+    X(343, 7, 7, 7)          \
+    X(384, 8, 8, 6)          \
+    X(392, 8, 7, 7)          \
+    X(448, 8, 8, 7)          \
+    X(512, 8, 8, 8)
+#define BDDA_PLAN_TABLE_3(X) \
+    X(576, 8, 8, 3, 3)       \
+    X(640, 8, 8, 5, 2)       \
+    X(720, 8, 6, 5, 3)       \
+    X(784, 8, 7, 7, 2)       \
+    X(896, 8, 8, 7, 2)       \
+    X(960, 8, 8, 5, 3)       \
     X(1024, 8, 8, 8, 2)
+// (the four groups are compiled in separate translation units, mbt_k_*.hip)
+#define BDDA_PLAN_TABLE(X) BDDA_PLAN_TABLE_0(X) BDDA_PLAN_TABLE_1(X) BDDA_PLAN_TABLE_2(X) BDDA_PLAN_TABLE_3(X)
 
 // calls f(Radices<...>{}) for the plan of size P; false when P has none
 template <class F>
@@ -595,6 +600,95 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
 }
 
 // ---------------------------------------------------------------- launchers
+// Each planned pass is launched through a per-plan function template
+// (*_plan_launch<R, PL, ...>).  Their explicit instantiations, with the
+// kernels they launch, live in the translation units mbt_k_<precision>_<g>.hip,
+// one per group of the plan table, so the kernels compile in parallel; every
+// other unit sees extern declarations (end of this file) and only dispatches.
+
+template <typename R, class PL, bool INV>
+void lines_plan_launch(const cplx<R>* in, cplx<R>* out, int nlin, int nlout, int64_t Zdim, int64_t pitch,
+                       int64_t outer, const cplx<R>* tw, hipStream_t st, const unsigned char* lmask, int nxm)
+{
+    using T = PlanTiles<R>;
+    const int64_t ntiles = ceil_div(Zdim, T::NQ_Y);
+    const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_Y);
+    constexpr int NT = plan_threads<PL, T::NQ_Y, 4, 256>();
+    hipLaunchKernelGGL((k_lines_plan<R, PL, INV, NT, T::NQ_Y>), dim3((unsigned)(ntiles * outer)), dim3(NT), shm, st,
+                       in, out, nlin, nlout, Zdim, pitch, ntiles, tw, lmask, nxm);
+    BDDA_HIP(hipGetLastError());
+}
+
+// false: the grid's lane offsets do not fit 32 bits (the all-in-LDS kernel takes it)
+template <typename R, class PL, int F>
+bool xfused_plan_launch(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, int Pzs, const cplx<R>* tw, int s,
+                        hipStream_t st)
+{
+    using T = PlanTiles<R>;
+    if constexpr (F * PL::P > MBT_MAX_PF) {
+        return false;  // refused earlier by mbt_padded_dims; never launched
+    } else {
+        // per-lane byte offsets into V (within a row block) and G^ are 32-bit
+        const uint64_t gh_bytes = (uint64_t)(d.Px / 2 + 1) * (d.Py / 2 + 1) * (d.Pz / 2 + 1) * (F == 3 ? 6 : 9) *
+                                  sizeof(cplx<R>);
+        if ((uint64_t)(PL::P / PL::r[0]) * d.Py * Pzs * sizeof(cplx<R>) >= (1ull << 32) || gh_bytes >= (1ull << 32))
+            return false;
+        // (odd line counts that fill the waves, 9 or 5 at 343, measured 19 % and
+        // 14 % slower here: their row segments straddle 64-byte boundaries)
+        constexpr int NQ = T::NQ_X;
+        const int ntz = (int)ceil_div(d.Pz, NQ);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * NQ);
+        constexpr int NT = plan_threads<PL, NQ, 2, 512>();
+        hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, NQ>), dim3((unsigned)((int64_t)s * d.Py * ((ntz + 1) & ~1))),
+                           dim3(NT), shm, st, V, gh, d, Pzs, tw, ntz, s);
+        BDDA_HIP(hipGetLastError());
+        return true;
+    }
+}
+
+template <typename R, class PL, int F>
+bool zfwd_plan_launch(const cplx<R>* X, int64_t ldx, const int* c2o, cplx<R>* U, const MbtDims& d, int Pzs,
+                      const cplx<R>* tw, int s, hipStream_t st, const int* lines, int64_t nlines)
+{
+    using T = PlanTiles<R>;
+    if constexpr (F * PL::P > MBT_MAX_PF) {
+        return false;  // refused earlier by mbt_padded_dims; never launched
+    } else {
+        constexpr int TL = T::TL_Z;  // more lines would cut the resident groups (LDS gather rows)
+        const int64_t nxny = (int64_t)d.nx * d.ny;
+        const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
+        if (ntiles == 0) return true;
+        const size_t shm = sizeof(cplx<R>) * (PL::P + zpass_lds_elems<PL, F, TL, T::NCX_Z>());
+        constexpr int NT = plan_threads<PL, TL, 2, 256>();
+        hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
+                           X, ldx, c2o, U, d, Pzs, tw, ntiles, s, lines, nlines);
+        BDDA_HIP(hipGetLastError());
+        return true;
+    }
+}
+
+template <typename R, class PL, int F>
+bool zinv_plan_launch(const cplx<R>* U, const int* c2o, const cplx<R>* X, int64_t ldx, const cplx<R>* diag,
+                      const cplx<R>* jpre, cplx<R>* Y, int64_t ldy, const MbtDims& d, int Pzs, const cplx<R>* tw,
+                      int s, hipStream_t st, const int* lines, int64_t nlines)
+{
+    using T = PlanTiles<R>;
+    if constexpr (F * PL::P > MBT_MAX_PF) {
+        return false;  // refused earlier by mbt_padded_dims; never launched
+    } else {
+        constexpr int TL = plan_lines<PL, T::TL_Z, 2, 256>();
+        const int64_t nxny = (int64_t)d.nx * d.ny;
+        const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
+        if (ntiles == 0) return true;
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)T::NCX_Z * TL * PL::P);
+        constexpr int NT = plan_threads<PL, TL, 2, 256>();
+        hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm, st,
+                           U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles, s, lines, nlines);
+        BDDA_HIP(hipGetLastError());
+        return true;
+    }
+}
+
 // the planned y kernel serves (P, pitch): a plan exists and its lane offsets fit 32 bits
 template <typename R>
 inline bool lines_plan_ok(int P, int64_t pitch)
@@ -607,16 +701,9 @@ inline bool launch_lines_plan(const cplx<R>* in, cplx<R>* out, int nlin, int nlo
                               int64_t outer, int P, const cplx<R>* tw, hipStream_t st,
                               const unsigned char* lmask = nullptr, int nxm = 1)
 {
-    using T = PlanTiles<R>;
     if (!lines_plan_ok<R>(P, pitch)) return false;
     return with_plan(P, [&](auto pl) {
-        using PL = decltype(pl);
-        const int64_t ntiles = ceil_div(Zdim, T::NQ_Y);
-        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_Y);
-        constexpr int NT = plan_threads<PL, T::NQ_Y, 4, 256>();
-        hipLaunchKernelGGL((k_lines_plan<R, PL, INV, NT, T::NQ_Y>), dim3((unsigned)(ntiles * outer)), dim3(NT),
-                           shm, st, in, out, nlin, nlout, Zdim, pitch, ntiles, tw, lmask, nxm);
-        BDDA_HIP(hipGetLastError());
+        lines_plan_launch<R, decltype(pl), INV>(in, out, nlin, nlout, Zdim, pitch, outer, tw, st, lmask, nxm);
     });
 }
 
@@ -624,32 +711,9 @@ template <typename R, int F>
 inline bool launch_xfused_plan(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, int Pzs, const cplx<R>* tw, int s,
                                hipStream_t st)
 {
-    using T = PlanTiles<R>;
     bool fits = true;
     const bool planned = with_xplan<R>(d.Px, [&](auto pl) {
-        using PL = decltype(pl);
-        // per-lane byte offsets into V (within a row block) and G^ are 32-bit
-        // (the all-in-LDS kernels take grids beyond that)
-        const uint64_t gh_bytes = (uint64_t)(d.Px / 2 + 1) * (d.Py / 2 + 1) * (d.Pz / 2 + 1) * (F == 3 ? 6 : 9) *
-                                  sizeof(cplx<R>);
-        if ((uint64_t)(PL::P / PL::r[0]) * d.Py * Pzs * sizeof(cplx<R>) >= (1ull << 32) || gh_bytes >= (1ull << 32)) {
-            fits = false;
-            return;
-        }
-        // (odd line counts that fill the waves, 9 or 5 at 343, measured 19 % and
-        // 14 % slower here: their row segments straddle 64-byte boundaries)
-        if constexpr (F * PL::P > MBT_MAX_PF) {
-            fits = false;  // refused earlier by mbt_padded_dims; not instantiated
-        } else {
-            constexpr int NQ = T::NQ_X;
-            const int ntz = (int)ceil_div(d.Pz, NQ);
-            const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * NQ);
-            constexpr int NT = plan_threads<PL, NQ, 2, 512>();
-            hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, NQ>),
-                               dim3((unsigned)((int64_t)s * d.Py * ((ntz + 1) & ~1))), dim3(NT), shm, st, V, gh, d,
-                               Pzs, tw, ntz, s);
-            BDDA_HIP(hipGetLastError());
-        }
+        fits = xfused_plan_launch<R, decltype(pl), F>(V, gh, d, Pzs, tw, s, st);
     });
     return planned && fits;
 }
@@ -670,24 +734,10 @@ inline bool launch_zfwd_plan(const cplx<R>* X, int64_t ldx, const int* c2o, cplx
                              const cplx<R>* tw, int s, hipStream_t st, const int* lines = nullptr,
                              int64_t nlines = 0)
 {
-    using T = PlanTiles<R>;
     if (!zplan_ok<R, F>(d, Pzs)) return false;
     bool fits = true;
     const bool planned = with_plan(d.Pz, [&](auto pl) {
-        using PL = decltype(pl);
-        if constexpr (F * PL::P > MBT_MAX_PF) {
-            fits = false;  // refused earlier by mbt_padded_dims; not instantiated
-        } else {
-            constexpr int TL = T::TL_Z;  // more lines would cut the resident groups (LDS gather rows)
-            const int64_t nxny = (int64_t)d.nx * d.ny;
-            const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
-            if (ntiles == 0) return;
-            const size_t shm = sizeof(cplx<R>) * (PL::P + zpass_lds_elems<PL, F, TL, T::NCX_Z>());
-            constexpr int NT = plan_threads<PL, TL, 2, 256>();
-            hipLaunchKernelGGL((k_zfwd_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm,
-                               st, X, ldx, c2o, U, d, Pzs, tw, ntiles, s, lines, nlines);
-            BDDA_HIP(hipGetLastError());
-        }
+        fits = zfwd_plan_launch<R, decltype(pl), F>(X, ldx, c2o, U, d, Pzs, tw, s, st, lines, nlines);
     });
     return planned && fits;
 }
@@ -698,26 +748,51 @@ inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X,
                              const cplx<R>* tw, int s, hipStream_t st, const int* lines = nullptr,
                              int64_t nlines = 0)
 {
-    using T = PlanTiles<R>;
     if (!zplan_ok<R, F>(d, Pzs)) return false;
     bool fits = true;
     const bool planned = with_plan(d.Pz, [&](auto pl) {
-        using PL = decltype(pl);
-        if constexpr (F * PL::P > MBT_MAX_PF) {
-            fits = false;  // refused earlier by mbt_padded_dims; not instantiated
-        } else {
-            constexpr int TL = plan_lines<PL, T::TL_Z, 2, 256>();
-            const int64_t nxny = (int64_t)d.nx * d.ny;
-            const int64_t ntiles = ceil_div(lines ? nlines : nxny, TL);
-            if (ntiles == 0) return;
-            const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)T::NCX_Z * TL * PL::P);
-            constexpr int NT = plan_threads<PL, TL, 2, 256>();
-            hipLaunchKernelGGL((k_zinv_plan<R, PL, F, NT, TL, T::NCX_Z>), dim3((unsigned)(ntiles * s)), dim3(NT), shm,
-                               st, U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, ntiles, s, lines, nlines);
-            BDDA_HIP(hipGetLastError());
-        }
+        fits = zinv_plan_launch<R, decltype(pl), F>(U, c2o, X, ldx, diag, jpre, Y, ldy, d, Pzs, tw, s, st, lines,
+                                                    nlines);
     });
     return planned && fits;
 }
+
+// ---------------------------------------------------------------- instantiations
+// BDDA_PLAN_INST(EXT, R, PL): the per-plan launch functions of one plan in
+// one precision, declared (EXT = extern) or defined (EXT empty).
+#define BDDA_PLAN_INST_X(EXT, R, PL, F)                                                                         \
+    EXT template bool xfused_plan_launch<R, PL, F>(cplx<R>*, const cplx<R>*, const MbtDims&, int, const cplx<R>*, \
+                                                   int, hipStream_t);
+#define BDDA_PLAN_INST_XZ(EXT, R, PL, F)                                                                        \
+    BDDA_PLAN_INST_X(EXT, R, PL, F)                                                                            \
+    EXT template bool zfwd_plan_launch<R, PL, F>(const cplx<R>*, int64_t, const int*, cplx<R>*, const MbtDims&,   \
+                                                 int, const cplx<R>*, int, hipStream_t, const int*, int64_t);   \
+    EXT template bool zinv_plan_launch<R, PL, F>(const cplx<R>*, const int*, const cplx<R>*, int64_t,            \
+                                                 const cplx<R>*, const cplx<R>*, cplx<R>*, int64_t,             \
+                                                 const MbtDims&, int, const cplx<R>*, int, hipStream_t,          \
+                                                 const int*, int64_t);
+#define BDDA_PLAN_INST(EXT, R, PL)                                                                               \
+    EXT template void lines_plan_launch<R, PL, false>(const cplx<R>*, cplx<R>*, int, int, int64_t, int64_t,      \
+                                                      int64_t, const cplx<R>*, hipStream_t,                    \
+                                                      const unsigned char*, int);                              \
+    EXT template void lines_plan_launch<R, PL, true>(const cplx<R>*, cplx<R>*, int, int, int64_t, int64_t,       \
+                                                     int64_t, const cplx<R>*, hipStream_t, const unsigned char*, \
+                                                     int);                                                      \
+    BDDA_PLAN_INST_XZ(EXT, R, PL, 3)                                                                           \
+    BDDA_PLAN_INST_XZ(EXT, R, PL, 6)
+
+// named plan types (the radix lists carry commas, which macro arguments cannot)
+#define BDDA_PLAN_TYPE(p, ...) using PlanOf##p = Radices<__VA_ARGS__>;
+BDDA_PLAN_TABLE(BDDA_PLAN_TYPE)
+#undef BDDA_PLAN_TYPE
+using PlanX192F32 = Radices<8, 8, 3>;  // with_xplan's FP32 x plan at 192
+
+#ifndef BDDA_PLAN_INSTANTIATE
+#define BDDA_PLAN_EXTERN(p, ...) BDDA_PLAN_INST(extern, float, PlanOf##p) BDDA_PLAN_INST(extern, double, PlanOf##p)
+BDDA_PLAN_TABLE(BDDA_PLAN_EXTERN)
+#undef BDDA_PLAN_EXTERN
+BDDA_PLAN_INST_X(extern, float, PlanX192F32, 3)
+BDDA_PLAN_INST_X(extern, float, PlanX192F32, 6)
+#endif
 
 }  // namespace bdda
