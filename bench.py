@@ -268,6 +268,47 @@ def sphere_leg(pkg):
     return out
 
 
+def configs_leg(pkg):
+    """The other stated workloads, measured as one solve each after a warm-up
+    solve (device buffers, operator cached): the 1024-monomer aggregate (8
+    dipoles across, m = 1.95 + 0.79i, 550 nm, s = 32, block COCG-rq-Jacobi)
+    and the 4096-monomer sweep aggregate (6 dipoles across, s = 8, block
+    BiCGGR-Jacobi) at 300, 650 and 1000 nm, FP64 and mixed, true residual
+    <= 1e-5."""
+    WL = importlib.import_module("block_dda_amd.workloads")
+    L = pkg.lib()
+    out = []
+    cases = [("agg1024", WL.build("agg1024"), pkg.COCG_RQ)]
+    first = WL.sweep(300.0)
+    cases += [("agg4096 sweep %g nm" % w, first if w == 300.0 else WL.sweep(w, target=first.target), pkg.BICGGR)
+              for w in (300.0, 650.0, 1000.0)]
+    for name, W, method in cases:
+        t = W.target
+        s = W.B.shape[1]
+        rows = 3 * t.nocc
+        op = pkg.MBT_fft_init(t.n, 3, t.lf, W.k)
+        op.set_target(t.address, W.diag)
+        dB, dX = ctypes.c_void_p(), ctypes.c_void_p()
+        pkg._check(L.bdda_dev_alloc(rows * s * 16, ctypes.byref(dB)))
+        pkg._check(L.bdda_dev_alloc(rows * s * 16, ctypes.byref(dX)))
+        Bh = np.asfortranarray(W.B)
+        pkg._check(L.bdda_memcpy_h2d(dB, Bh.ctypes.data, rows * s * 16))
+        row = {"workload": name, "nocc": t.nocc, "n": list(t.n), "padded": list(op.grid), "s": s,
+               "method": METHOD_NAMES[{pkg.COCG_RQ: "cocg_rq", pkg.BICGGR: "bicggr"}[method]]}
+        for mname, mode in (("fp64", pkg.MODE_FP64), ("mixed", pkg.MODE_MIXED)):
+            solve_dev(pkg, op, dB, dX, s, 1e-5, 3000, mode, method)
+            r = solve_dev(pkg, op, dB, dX, s, 1e-5, 3000, mode, method)
+            row[mname] = {"solve_s": r["wall_s"], "iters": r["iters"], "matvecs": r["matvecs"],
+                          "true_res": r["true_res"], "converged": r["converged"]}
+        row["speedup_mixed_vs_fp64"] = row["fp64"]["solve_s"] / row["mixed"]["solve_s"]
+        log("config %s" % row)
+        out.append(row)
+        L.bdda_dev_free(dB)
+        L.bdda_dev_free(dX)
+        op.close()
+    return out
+
+
 def sweep_leg(pkg, nwl, method, workers=4, sequential=True):
     """The wavelength sweep on the 4096-monomer aggregate (6 dipoles across,
     s = 8, m(lambda) linear from 2.0 + 1.0i at 300 nm to 2.4 + 0.5i at
@@ -465,6 +506,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-sphere", action="store_true", help="skip the sphere-against-Mie leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other workloads (1024-monomer s = 32, sweep aggregate at three wavelengths)")
     ap.add_argument("--sweep-workers", type=int, default=4, help="serving-loop workers for the sweep")
     ap.add_argument("--sweep-served-only", action="store_true", help="skip the sweep's one-by-one solves")
     ap.add_argument("--sweep", type=int, default=0,
@@ -645,6 +688,7 @@ def main():
                            workers=args.serving_workers)
                if args.serving_requests > 0 else None)
     sphere = None if args.no_sphere else sphere_leg(pkg)
+    other = None if args.no_configs else configs_leg(pkg)
     refpub = reference_leg(pkg, ref_sizes) if ref_sizes else None
     cpu = None
     if not args.no_cpu:
@@ -683,6 +727,7 @@ def main():
                         "pass_GBs": v["pass_GBs"]} for kk, v in mv.items()},
         "roofline": roofline,
         "sphere_vs_mie": sphere,
+        "other_configs": other,
         "reference_published": refpub,
         "cpu_baseline": cpu,
     }
