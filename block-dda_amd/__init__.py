@@ -85,6 +85,9 @@ SIGNATURES = {
     "bdda_hbm_probe": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _dp, _dp, _dp]),
     "bdda_dev_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "bdda_dev_free": (ctypes.c_int, [_vp]),
+    "bdda_dev_trim": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
+    "bdda_dev_pool_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                                           ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "bdda_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "bdda_memcpy_d2h": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "bdda_memcpy_d2d": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
@@ -352,6 +355,25 @@ def lu_solve(A, B, full=False):
     X = np.zeros_like(B2, order="F")
     _check(lib().bdda_lu_solve(_ptr(A), _ptr(B2), _ptr(X), s, B2.shape[1], int(bool(full))))
     return X.reshape(B.shape, order="F")
+
+
+def dev_pool_stats():
+    """The device allocator's bytes held from the runtime and in live blocks,
+    runtime allocations made and requests served from idle blocks (current
+    device)."""
+    held, used = ctypes.c_size_t(), ctypes.c_size_t()
+    nalloc, reuse = ctypes.c_int64(), ctypes.c_int64()
+    _check(lib().bdda_dev_pool_stats(ctypes.byref(held), ctypes.byref(used), ctypes.byref(nalloc),
+                                     ctypes.byref(reuse)))
+    return {"held": held.value, "in_use": used.value, "runtime_allocs": nalloc.value, "reuses": reuse.value}
+
+
+def dev_trim():
+    """Give the device allocator's wholly idle memory back to the runtime;
+    returns the bytes released."""
+    n = ctypes.c_size_t()
+    _check(lib().bdda_dev_trim(ctypes.byref(n)))
+    return n.value
 
 
 def qr_reduced(A):

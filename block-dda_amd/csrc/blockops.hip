@@ -21,30 +21,38 @@ constexpr int MAX_PARTIAL_BLOCKS = 1024;
 constexpr int COEF_SLOTS = 64;
 constexpr int COEF_SLOT_ELEMS = 2 * 64 * 64;
 
-BlockWork::~BlockWork()
+BlockWork::~BlockWork() { release(); }
+
+void BlockWork::release()
 {
-    if (partial) (void)hipFree(partial);
-    if (red) (void)hipFree(red);
-    if (coef) (void)hipFree(coef);
+    dev_free(partial, stream);
+    dev_free(red, stream);
+    dev_free(coef, stream);
+    partial = red = coef = nullptr;
+    partial_elems = red_elems = 0;
     if (host_pinned) (void)hipHostFree(host_pinned);
+    host_pinned = nullptr;
+    host_elems = 0;
 }
 
 void BlockWork::reserve(size_t partial_n, size_t red_n)
 {
     if (partial_n > partial_elems) {
-        if (partial) BDDA_HIP(hipFree(partial));
+        dev_free(partial, stream);
         partial = nullptr;
-        BDDA_HIP(hipMalloc(&partial, sizeof(c64) * partial_n));
+        partial_elems = 0;
+        partial = dev_alloc_n<c64>(partial_n);
         partial_elems = partial_n;
     }
     if (red_n > red_elems) {
-        if (red) BDDA_HIP(hipFree(red));
+        dev_free(red, stream);
         red = nullptr;
-        BDDA_HIP(hipMalloc(&red, sizeof(c64) * red_n));
+        red_elems = 0;
+        red = dev_alloc_n<c64>(red_n);
         red_elems = red_n;
     }
     if (!coef) {
-        BDDA_HIP(hipMalloc(&coef, sizeof(c64) * COEF_SLOTS * COEF_SLOT_ELEMS));
+        coef = dev_alloc_n<c64>((size_t)COEF_SLOTS * COEF_SLOT_ELEMS);
         BDDA_HIP(hipHostMalloc(&host_pinned, sizeof(c64) * (COEF_SLOTS * COEF_SLOT_ELEMS + 64 * 64),
                                hipHostMallocDefault));
         host_elems = COEF_SLOTS * COEF_SLOT_ELEMS;

@@ -7,6 +7,7 @@
 // FP64 where the reference's Eigen LU / products run (host side, FP64).
 #pragma once
 #include "bdda_common.hpp"
+#include "devmem.hpp"
 #include <vector>
 
 namespace bdda {
@@ -25,6 +26,9 @@ struct BlockWork {
     int coef_slot = 0;         // rotating staging slot for combo coefficients
     ~BlockWork();
     void reserve(size_t partial_n, size_t red_n);
+    // device buffers back to the allocator (after the stream's work); call
+    // before the stream is destroyed
+    void release();
 };
 
 // C(a x b) = op(A)^ B with op = transpose (conjA = 0) or adjoint (conjA = 1),

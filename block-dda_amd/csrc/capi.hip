@@ -28,6 +28,9 @@ struct bdda_mbt {
     int have_target = 0;
     ~bdda_mbt()
     {
+        // device buffers go back to the allocator after the stream's work,
+        // so the stream is destroyed last
+        prob.release();
         if (op) mbt_destroy(op);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -131,15 +134,9 @@ static void apply_host(bdda_mbt* h, const bdda_c64* x, int64_t ldx, bdda_c64* y,
                        int jacobi)
 {
     const int64_t rows = occupied ? h->prob.rows() : (int64_t)h->f * h->n[0] * h->n[1] * h->n[2];
-    cplx<R>* dx = nullptr;
-    cplx<R>* dy = nullptr;
-    BDDA_HIP(hipMalloc(&dx, sizeof(cplx<R>) * rows * ncols));
-    struct Free {
-        void* p;
-        ~Free() { if (p) (void)hipFree(p); }
-    } fx{dx};
-    BDDA_HIP(hipMalloc(&dy, sizeof(cplx<R>) * rows * ncols));
-    Free fy{dy};
+    DevPtr<cplx<R>> bx((size_t)rows * ncols, h->stream), by((size_t)rows * ncols, h->stream);
+    cplx<R>* dx = bx.p;
+    cplx<R>* dy = by.p;
     to_dev<R>(x, ldx, rows, ncols, dx, h->stream);
     if (occupied)
         mbt_apply_occupied<R>(h->op, dx, rows, dy, rows, ncols, h->prob.diag<R>(), jacobi ? h->prob.jpre<R>() : nullptr,
@@ -239,14 +236,10 @@ static int sm_dev(bdda_mbt* h, const c64* dX, int npairs, const double* u, const
     return BDDA_OK;
 }
 
+// a device array of at least one element, freed after the work on stream st
 template <typename T>
-struct DevMem {
-    T* p = nullptr;
-    explicit DevMem(size_t n) { BDDA_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1))); }
-    ~DevMem()
-    {
-        if (p) (void)hipFree(p);
-    }
+struct DevMem : DevPtr<T> {
+    DevMem(size_t n, hipStream_t st) : DevPtr<T>(std::max<size_t>(n, 1), st) {}
 };
 
 extern "C" {
@@ -414,15 +407,9 @@ int bdda_solve(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64* B,
     if (ldb < rows || ldx < rows) return fail(BDDA_EINVAL, "leading dimension smaller than f*nocc");
     if (L < 1 || L > 64) return fail(BDDA_EINVAL, "number of right-hand sides must be in 1..64");
     BDDA_TRY({
-        c64* dB = nullptr;
-        c64* dX = nullptr;
-        BDDA_HIP(hipMalloc(&dB, sizeof(c64) * rows * std::max(L, 1)));
-        struct Free {
-            void* p;
-            ~Free() { if (p) (void)hipFree(p); }
-        } fb{dB};
-        BDDA_HIP(hipMalloc(&dX, sizeof(c64) * rows * std::max(L, 1)));
-        Free fx{dX};
+        DevMem<c64> bB((size_t)rows * L, h->stream), bX((size_t)rows * L, h->stream);
+        c64* dB = bB.p;
+        c64* dX = bX.p;
         to_dev<double>(B, ldb, rows, L, dB, h->stream);
         int rc = solve_common(h, method, jacobi, mode, dB, L, tol, itermax, dX, info, hist, hist_len);
         if (rc == BDDA_OK || rc == BDDA_ENOCONV) from_dev<double>(dX, rows, L, X, ldx, h->stream);
@@ -456,8 +443,8 @@ int bdda_residual_columns(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdd
         if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
         const int64_t rows = h->prob.rows();
         if (L < 1 || ldb < rows || ldx < rows) return fail(BDDA_EINVAL, "bad block shape");
-        DevMem<c64> dB((size_t)rows * L);
-        DevMem<c64> dX((size_t)rows * L);
+        DevMem<c64> dB((size_t)rows * L, h->stream);
+        DevMem<c64> dX((size_t)rows * L, h->stream);
         to_dev<double>(B, ldb, rows, L, dB.p, h->stream);
         to_dev<double>(X, ldx, rows, L, dX.p, h->stream);
         return rescol_dev(h, dB.p, dX.p, L, res);
@@ -478,8 +465,8 @@ int bdda_cross_sections(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdda_
         if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
         const int64_t rows = h->prob.rows();
         if (L < 1 || ldb < rows || ldx < rows) return fail(BDDA_EINVAL, "bad block shape");
-        DevMem<c64> dB((size_t)rows * L);
-        DevMem<c64> dX((size_t)rows * L);
+        DevMem<c64> dB((size_t)rows * L, h->stream);
+        DevMem<c64> dX((size_t)rows * L, h->stream);
         to_dev<double>(B, ldb, rows, L, dB.p, h->stream);
         to_dev<double>(X, ldx, rows, L, dX.p, h->stream);
         return xs_dev(h, dB.p, dX.p, L, alpha_mode, Cext, Cabs, Csca);
@@ -522,7 +509,7 @@ int bdda_scattering_matrix(bdda_mbt* h, const bdda_c64* X, int64_t ldx, int npai
         if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
         const int64_t rows = h->prob.rows();
         if (npairs < 1 || ldx < rows) return fail(BDDA_EINVAL, "bad block shape");
-        DevMem<c64> dX((size_t)rows * 2 * npairs);
+        DevMem<c64> dX((size_t)rows * 2 * npairs, h->stream);
         to_dev<double>(X, ldx, rows, 2 * npairs, dX.p, h->stream);
         return sm_dev(h, dX.p, npairs, u, e1, nang, S11, S12, S);
     })
@@ -579,16 +566,17 @@ int bdda_qr_reduced(const bdda_c64* A, int64_t m, int n, bdda_c64* Q, bdda_c64* 
     BDDA_TRY({
         BlockWork w;
         BDDA_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
-        struct S {
-            hipStream_t s;
-            ~S() { (void)hipStreamDestroy(s); }
-        } sg{w.stream};
-        c64* dA = nullptr;
-        BDDA_HIP(hipMalloc(&dA, sizeof(c64) * m * n));
-        struct Free {
-            void* p;
-            ~Free() { if (p) (void)hipFree(p); }
-        } fa{dA};
+        struct S {  // w's buffers go back after its stream's work, then the stream goes
+            BlockWork& w;
+            ~S()
+            {
+                w.release();
+                (void)hipStreamDestroy(w.stream);
+                w.stream = nullptr;
+            }
+        } sg{w};
+        DevPtr<c64> bA((size_t)m * n, w.stream);
+        c64* dA = bA.p;
         BDDA_HIP(hipMemcpy(dA, A, sizeof(c64) * m * n, hipMemcpyHostToDevice));
         HostMat Rm;
         int rc = qr_mgs2<double>(w, dA, m, dA, m, m, n, Rm);
@@ -611,12 +599,8 @@ int bdda_lu_solve(const bdda_c64* A, const bdda_c64* B, bdda_c64* X, int s, int 
             ~S() { (void)hipStreamDestroy(s); }
         } sg{st};
         const size_t na = (size_t)s * s, nx = (size_t)s * nb;
-        c64* d = nullptr;
-        BDDA_HIP(hipMalloc(&d, sizeof(c64) * (na + 2 * nx + 1)));
-        struct Free {
-            void* p;
-            ~Free() { if (p) (void)hipFree(p); }
-        } fd{d};
+        DevPtr<c64> bd(na + 2 * nx + 1, st);
+        c64* d = bd.p;
         c64 *dA = d, *dB = d + na, *dX = dB + nx;
         int* flag = reinterpret_cast<int*>(dX + nx);
         BDDA_HIP(hipMemcpy(dA, A, sizeof(c64) * na, hipMemcpyHostToDevice));
@@ -646,15 +630,35 @@ int bdda_hbm_probe(void* a, void* b, size_t bytes, double* copy_gbs, double* rea
 int bdda_dev_alloc(size_t bytes, void** p)
 {
     if (!p) return fail(BDDA_EINVAL, "null pointer");
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? BDDA_ENOMEM : BDDA_EHIP, hipGetErrorString(e));
-    return BDDA_OK;
+    *p = nullptr;
+    BDDA_TRY({
+        *p = dev_malloc(bytes);
+        return BDDA_OK;
+    })
 }
 
 int bdda_dev_free(void* p)
 {
-    hipError_t e = hipFree(p);
-    return e == hipSuccess ? BDDA_OK : fail(BDDA_EHIP, hipGetErrorString(e));
+    // waits for the whole device (a caller's buffer may be in use on any stream)
+    dev_free(p, nullptr);
+    return BDDA_OK;
+}
+
+int bdda_dev_trim(size_t* released)
+{
+    const size_t n = dev_trim();
+    if (released) *released = n;
+    return BDDA_OK;
+}
+
+int bdda_dev_pool_stats(size_t* held, size_t* in_use, int64_t* runtime_allocs, int64_t* reuses)
+{
+    const DevStats s = dev_stats();
+    if (held) *held = s.held;
+    if (in_use) *in_use = s.in_use;
+    if (runtime_allocs) *runtime_allocs = s.runtime_allocs;
+    if (reuses) *reuses = s.reuses;
+    return BDDA_OK;
 }
 
 int bdda_memcpy_h2d(void* dst, const void* src, size_t bytes)

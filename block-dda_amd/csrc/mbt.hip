@@ -12,6 +12,7 @@
 // convolution over the cuboid (no index digit borrows); this file computes
 // that convolution with 3-D FFTs on a smooth padded grid instead.
 #include "mbt_passes.hpp"
+#include "devmem.hpp"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -183,7 +184,7 @@ static void upload_twiddles(int P, void** dst)
         const long double a = -2.0L * 3.14159265358979323846264338327950288L * q / P;
         h[q] = mk((R)std::cos(a), (R)std::sin(a));
     }
-    BDDA_HIP(hipMalloc(dst, sizeof(cplx<R>) * P));
+    *dst = dev_alloc_n<cplx<R>>(P);
     BDDA_HIP(hipMemcpy(*dst, h.data(), sizeof(cplx<R>) * P, hipMemcpyHostToDevice));
 }
 
@@ -221,10 +222,10 @@ MbtOperator* mbt_create(const int n[3], int f, double lf, double kre, double kim
         // full FP64 kernel grid, forward 3-D FFT, octant extraction
         const int64_t G = (int64_t)d.Px * d.Py * d.Pz;
         const int nc = op->nc;
-        c64* full = nullptr;
-        c64* tmp = nullptr;
-        BDDA_HIP(hipMalloc(&full, sizeof(c64) * G * nc));
-        BDDA_HIP(hipMalloc(&tmp, sizeof(c64) * G * nc));
+        // the build transient (two full FP64 kernel grids), freed below
+        DevPtr<c64> fullb((size_t)G * nc, stream), tmpb((size_t)G * nc, stream);
+        c64* full = fullb.p;
+        c64* tmp = tmpb.p;
         const double scale = 1.0 / (double)G;
         hipLaunchKernelGGL(k_green_eval, dim3(grid_for(G)), dim3(256), 0, stream, full, d, nc, lf, kre, kim, scale);
         BDDA_HIP(hipGetLastError());
@@ -238,14 +239,12 @@ MbtOperator* mbt_create(const int n[3], int f, double lf, double kre, double kim
         launch_lines<double, false>(full, tmp, d.Px, d.Px, (int64_t)d.Py * d.Pz * nc, (int64_t)d.Py * d.Pz * nc, 1,
                                     op->ax[0], tw[0], stream);
         op->gh_elems = (size_t)(d.Px / 2 + 1) * (d.Py / 2 + 1) * (d.Pz / 2 + 1);
-        if (want64) BDDA_HIP(hipMalloc(&op->gh64, sizeof(c64) * op->gh_elems * nc));
-        if (want32) BDDA_HIP(hipMalloc(&op->gh32, sizeof(c32) * op->gh_elems * nc));
+        if (want64) op->gh64 = dev_alloc_n<c64>(op->gh_elems * nc);
+        if (want32) op->gh32 = dev_alloc_n<c32>(op->gh_elems * nc);
         hipLaunchKernelGGL(k_reduce_gh, dim3(grid_for((int64_t)op->gh_elems * nc)), dim3(256), 0, stream, tmp,
                            (c64*)op->gh64, (c32*)op->gh32, d, nc);
         BDDA_HIP(hipGetLastError());
         BDDA_HIP(hipStreamSynchronize(stream));
-        BDDA_HIP(hipFree(full));
-        BDDA_HIP(hipFree(tmp));
         op->have64 = want64 ? 1 : 0;
         op->have32 = want32 ? 1 : 0;
         for (auto& e : op->ev) BDDA_HIP(hipEventCreate(&e));
@@ -259,17 +258,19 @@ MbtOperator* mbt_create(const int n[3], int f, double lf, double kre, double kim
 void mbt_destroy(MbtOperator* op)
 {
     if (!op) return;
+    // (the stream outlives the operator; its handle owns both)
+    hipStream_t st = op->stream;
     for (int a = 0; a < 3; ++a) {
-        if (op->tw64[a]) (void)hipFree(op->tw64[a]);
-        if (op->tw32[a]) (void)hipFree(op->tw32[a]);
+        dev_free(op->tw64[a], st);
+        dev_free(op->tw32[a], st);
     }
-    if (op->gh64) (void)hipFree(op->gh64);
-    if (op->gh32) (void)hipFree(op->gh32);
-    if (op->ws) (void)hipFree(op->ws);
-    if (op->c2o) (void)hipFree(op->c2o);
-    if (op->addr_dev) (void)hipFree(op->addr_dev);
-    if (op->lines) (void)hipFree(op->lines);
-    if (op->lmask) (void)hipFree(op->lmask);
+    dev_free(op->gh64, st);
+    dev_free(op->gh32, st);
+    dev_free(op->ws, st);
+    dev_free(op->c2o, st);
+    dev_free(op->addr_dev, st);
+    dev_free(op->lines, st);
+    dev_free(op->lmask, st);
     for (auto& e : op->ev)
         if (e) (void)hipEventDestroy(e);
     delete op;
@@ -279,14 +280,14 @@ void mbt_set_address(MbtOperator* op, const int* address_host, int64_t nocc)
 {
     const int64_t C = (int64_t)op->d.nx * op->d.ny * op->d.nz;
     if (!op->c2o) {
-        BDDA_HIP(hipMalloc(&op->c2o, sizeof(int) * C));
+        op->c2o = dev_alloc_n<int>(C);
         op->c2o_cells = C;
     }
     if (op->addr_n < nocc) {
-        if (op->addr_dev) BDDA_HIP(hipFree(op->addr_dev));
+        dev_free(op->addr_dev, op->stream);
         op->addr_dev = nullptr;
         op->addr_n = 0;
-        BDDA_HIP(hipMalloc(&op->addr_dev, sizeof(int) * std::max<int64_t>(nocc, 1)));
+        op->addr_dev = dev_alloc_n<int>(std::max<int64_t>(nocc, 1));
         op->addr_n = nocc;
     }
     BDDA_HIP(hipMemcpyAsync(op->addr_dev, address_host, sizeof(int) * nocc, hipMemcpyHostToDevice, op->stream));
@@ -301,11 +302,11 @@ void mbt_set_address(MbtOperator* op, const int* address_host, int64_t nocc)
     std::vector<int> ids;
     for (int64_t l = 0; l < nxny; ++l)
         if (mask[l]) ids.push_back((int)l);
-    if (!op->lmask) BDDA_HIP(hipMalloc(&op->lmask, nxny));
-    if (op->lines) BDDA_HIP(hipFree(op->lines));
+    if (!op->lmask) op->lmask = dev_alloc_n<unsigned char>(nxny);
+    dev_free(op->lines, op->stream);
     op->lines = nullptr;
     op->nlines = 0;
-    BDDA_HIP(hipMalloc(&op->lines, sizeof(int) * std::max<size_t>(ids.size(), 1)));
+    op->lines = dev_alloc_n<int>(std::max<size_t>(ids.size(), 1));
     BDDA_HIP(hipMemcpyAsync(op->lmask, mask.data(), nxny, hipMemcpyHostToDevice, op->stream));
     if (!ids.empty())
         BDDA_HIP(hipMemcpyAsync(op->lines, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, op->stream));
@@ -318,12 +319,11 @@ void mbt_ensure_workspace(MbtOperator* op, size_t bytes)
 {
     if (op->ws_bytes >= bytes) return;
     if (op->ws) {
-        BDDA_HIP(hipStreamSynchronize(op->stream));
-        BDDA_HIP(hipFree(op->ws));
+        dev_free(op->ws, op->stream);
         op->ws = nullptr;
         op->ws_bytes = 0;
     }
-    BDDA_HIP(hipMalloc(&op->ws, bytes));
+    op->ws = dev_malloc(bytes);
     op->ws_bytes = bytes;
 }
 

@@ -47,21 +47,17 @@ void cross_sections(Problem& P, const c64* dB, const c64* dX, int L, double k, i
 {
     const int64_t rows = P.rows();
     const int nblk = (int)std::min<int64_t>(256, std::max<int64_t>(1, ceil_div(rows, NT_OBS)));
-    double* dpart = nullptr;
-    BDDA_HIP(hipMalloc(&dpart, sizeof(double) * 3 * nblk * L));
+    DevPtr<double> dpb((size_t)3 * nblk * L, P.work.stream);
+    double* dpart = dpb.p;
     std::vector<double> part((size_t)3 * nblk * L);
-    try {
+    {
         hipLaunchKernelGGL(k_xs_partial, dim3(nblk, L), dim3(NT_OBS), 0, P.work.stream, dB, dX, P.diag64, rows, P.f,
                            alpha_each, 2.0 * k * k * k / 3.0, dpart);
         BDDA_HIP(hipGetLastError());
         BDDA_HIP(hipMemcpyAsync(part.data(), dpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
                                 P.work.stream));
         BDDA_HIP(hipStreamSynchronize(P.work.stream));
-    } catch (...) {
-        (void)hipFree(dpart);
-        throw;
     }
-    BDDA_HIP(hipFree(dpart));
     for (int l = 0; l < L; ++l) {
         double im = 0, ab = 0;
         for (int g = 0; g < nblk; ++g) {
@@ -254,9 +250,6 @@ void scattering_amplitudes(Problem& P, const c64* dX, int npairs, double k, cons
     const int64_t chunk = ceil_div(ceil_div(nocc, nchunk), TJ_FF) * TJ_FF;
     const int64_t n = (int64_t)npairs * nang * 4;
     FfGeom g{op->d.nx, op->d.ny, op->d.nz, op->lf, k, nang};
-    double* dgeo = nullptr;
-    c64* dpart = nullptr;
-    c64* dout = nullptr;
     std::vector<double> geo((size_t)6 * npairs);
     for (int p = 0; p < npairs; ++p)
         for (int q = 0; q < 3; ++q) {
@@ -264,16 +257,13 @@ void scattering_amplitudes(Problem& P, const c64* dX, int npairs, double k, cons
             geo[3 * npairs + 3 * p + q] = e1[3 * p + q];
         }
     std::vector<c64> red(n);
-    auto cleanup = [&]() {
-        if (dgeo) (void)hipFree(dgeo);
-        if (dpart) (void)hipFree(dpart);
-        if (dout) (void)hipFree(dout);
-    };
-    try {
-        BDDA_HIP(hipMalloc(&dgeo, sizeof(double) * geo.size()));
-        BDDA_HIP(hipMalloc(&dpart, sizeof(c64) * n * nchunk));
-        BDDA_HIP(hipMalloc(&dout, sizeof(c64) * n));
+    {
         hipStream_t st = P.work.stream;
+        DevPtr<double> dgeob(geo.size(), st);
+        DevPtr<c64> dpartb((size_t)n * nchunk, st), doutb((size_t)n, st);
+        double* dgeo = dgeob.p;
+        c64* dpart = dpartb.p;
+        c64* dout = doutb.p;
         BDDA_HIP(hipMemcpyAsync(dgeo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice, st));
         const dim3 grid(nchunk, npairs, (unsigned)ceil_div(nang, NT_OBS));
         if (P.f == 3)
@@ -287,11 +277,7 @@ void scattering_amplitudes(Problem& P, const c64* dX, int npairs, double k, cons
         BDDA_HIP(hipGetLastError());
         BDDA_HIP(hipMemcpyAsync(red.data(), dout, sizeof(c64) * n, hipMemcpyDeviceToHost, st));
         BDDA_HIP(hipStreamSynchronize(st));
-    } catch (...) {
-        cleanup();
-        throw;
     }
-    cleanup();
     // S = -ik * k^2 * projection: S2 = theta1, S4 = phi1, S3 = theta2, S1 = phi2
     const double k3 = k * k * k;
     for (int64_t e = 0; e < (int64_t)npairs * nang; ++e) {

@@ -27,26 +27,29 @@
 
 namespace bdda {
 
+// a solver's device block, freed after the work queued on its stream
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
-    explicit DevBuf(size_t count) : n(count)
+    hipStream_t st = nullptr;
+    DevBuf(size_t count, hipStream_t stream) : n(count), st(stream)
     {
-        if (count) BDDA_HIP(hipMalloc(&p, sizeof(T) * count));
+        if (count) p = dev_alloc_n<T>(count);
     }
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
+    ~DevBuf() { dev_free(p, st); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
 };
 
-Problem::~Problem()
+Problem::~Problem() { release(); }
+
+void Problem::release()
 {
-    for (void* p : {(void*)diag64, (void*)jpre64, (void*)diag32, (void*)jpre32})
-        if (p) (void)hipFree(p);
+    for (void* p : {(void*)diag64, (void*)jpre64, (void*)diag32, (void*)jpre32}) dev_free(p, work.stream);
+    diag64 = jpre64 = nullptr;
+    diag32 = jpre32 = nullptr;
+    work.release();
 }
 
 template <>
@@ -70,14 +73,13 @@ void problem_set_diag(Problem& P, const c64* diag_host)
         d32[i] = cvt<float>(d);
         j32[i] = cvt<float>(jp[i]);
     }
-    for (void* p : {(void*)P.diag64, (void*)P.jpre64, (void*)P.diag32, (void*)P.jpre32})
-        if (p) BDDA_HIP(hipFree(p));
+    for (void* p : {(void*)P.diag64, (void*)P.jpre64, (void*)P.diag32, (void*)P.jpre32}) dev_free(p, P.work.stream);
     P.diag64 = P.jpre64 = nullptr;
     P.diag32 = P.jpre32 = nullptr;
-    BDDA_HIP(hipMalloc(&P.diag64, sizeof(c64) * n));
-    BDDA_HIP(hipMalloc(&P.jpre64, sizeof(c64) * n));
-    BDDA_HIP(hipMalloc(&P.diag32, sizeof(c32) * n));
-    BDDA_HIP(hipMalloc(&P.jpre32, sizeof(c32) * n));
+    P.diag64 = dev_alloc_n<c64>(n);
+    P.jpre64 = dev_alloc_n<c64>(n);
+    P.diag32 = dev_alloc_n<c32>(n);
+    P.jpre32 = dev_alloc_n<c32>(n);
     BDDA_HIP(hipMemcpy(P.diag64, diag_host, sizeof(c64) * n, hipMemcpyHostToDevice));
     BDDA_HIP(hipMemcpy(P.jpre64, jp.data(), sizeof(c64) * n, hipMemcpyHostToDevice));
     BDDA_HIP(hipMemcpy(P.diag32, d32.data(), sizeof(c32) * n, hipMemcpyHostToDevice));
@@ -123,7 +125,7 @@ struct SmallState {
     DevBuf<c64> buf;
     c64* probe;  // [0].x: the value read per iteration; [1]: flag word
     int* flag;
-    SmallState(BlockWork& w_, int s_, int nmat, int64_t rows) : w(w_), s(s_), buf((size_t)nmat * s_ * s_ + 4)
+    SmallState(BlockWork& w_, int s_, int nmat, int64_t rows) : w(w_), s(s_), buf((size_t)nmat * s_ * s_ + 4, w_.stream)
     {
         probe = buf.p + (size_t)nmat * s * s;
         flag = reinterpret_cast<int*>(probe + 1);
@@ -174,7 +176,7 @@ int run_cocg_rq(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     const int s = c.s;
     const int64_t rows = c.rows, N = rows * s;
     const c64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-    DevBuf<cplx<R>> Bp(c.jacobi ? N : 0), Q(N), Qn(N), S(N), AS(N), T(N), scr(N);
+    DevBuf<cplx<R>> Bp(c.jacobi ? N : 0, w.stream), Q(N, w.stream), Qn(N, w.stream), S(N, w.stream), AS(N, w.stream), T(N, w.stream), scr(N, w.stream);
     const cplx<R>* Bu = B;
     if (c.jacobi) {  // B_pre = jpre .* B  (bl_cocg_rq_jacobi_mvp_fft.cpp:22-27)
         scale_rows<R>(w.stream, c.P.jpre<R>(), B, rows, Bp.p, rows, rows, s);
@@ -270,7 +272,7 @@ int run_bicgstab(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     const int s = c.s;
     const int64_t rows = c.rows, N = rows * s;
     const c64 one = mk(1.0, 0.0);
-    DevBuf<cplx<R>> Rb(N), Pb(N), Pn(N), R0(N), V(N), T(N), Z(N);
+    DevBuf<cplx<R>> Rb(N, w.stream), Pb(N, w.stream), Pn(N, w.stream), R0(N, w.stream), V(N, w.stream), T(N, w.stream), Z(N, w.stream);
     if (c.jacobi) scale_rows<R>(w.stream, c.P.jpre<R>(), B, rows, Rb.p, rows, rows, s);
     else copy_block<R>(w, B, Rb.p, rows, s);
     const double bnorm = norm_fro<R>(w, Rb.p, rows, rows, s);
@@ -349,7 +351,7 @@ int run_bicggr(Ctx& c, const cplx<R>* B, cplx<R>* X, double tol, int itermax)
     const int s = c.s;
     const int64_t rows = c.rows, N = rows * s;
     const c64 one = mk(1.0, 0.0);
-    DevBuf<cplx<R>> Rb(N), Rn(N), Pb(N), R0(N), V(N), W(N), U(N), Y(N);
+    DevBuf<cplx<R>> Rb(N, w.stream), Rn(N, w.stream), Pb(N, w.stream), R0(N, w.stream), V(N, w.stream), W(N, w.stream), U(N, w.stream), Y(N, w.stream);
     if (c.jacobi) scale_rows<R>(w.stream, c.P.jpre<R>(), B, rows, Rb.p, rows, rows, s);
     else copy_block<R>(w, B, Rb.p, rows, s);
     const double bnorm = norm_fro<R>(w, Rb.p, rows, rows, s);
@@ -451,7 +453,7 @@ double true_residual(Problem& P, const c64* B64, const c64* X64, int s)
 {
     const int64_t rows = P.rows(), N = rows * s;
     BlockWork& w = P.work;
-    DevBuf<c64> AX(N);
+    DevBuf<c64> AX(N, w.stream);
     problem_matvec<double>(P, X64, AX.p, s, 0);
     Combo<double> cb;
     cb.nv = 2;
@@ -467,7 +469,7 @@ void residual_columns(Problem& P, const c64* B64, const c64* X64, int s, double*
 {
     const int64_t rows = P.rows(), N = rows * s;
     BlockWork& w = P.work;
-    DevBuf<c64> AX(N);
+    DevBuf<c64> AX(N, w.stream);
     problem_matvec<double>(P, X64, AX.p, s, 0);
     Combo<double> cb;
     cb.nv = 2;
@@ -497,7 +499,7 @@ int solve_fp(Problem& P, int method, int jacobi, int mode, const c64* B64, c64* 
         rc = run_method<double>(c, method, B64, X64, tol, itermax);
         info.outer_iters = 1;
     } else if (mode == MODE_FP32) {
-        DevBuf<c32> B32(N), X32(N);
+        DevBuf<c32> B32(N, w.stream), X32(N, w.stream);
         convert_64_to_32(w.stream, B64, rows, B32.p, rows, rows, s, 1.0);
         rc = run_method<float>(c, method, B32.p, X32.p, tol, itermax);
         fill_zero<double>(w.stream, X64, rows, rows, s);
@@ -506,8 +508,8 @@ int solve_fp(Problem& P, int method, int jacobi, int mode, const c64* B64, c64* 
     } else if (mode == MODE_MIXED || mode == MODE_MIXED_COLUMNS) {
         // FP64 defect correction: r = B - A x (FP64 matvec), A d = r in FP32,
         // x += d, until ||B - A x|| / ||B|| <= tol.
-        DevBuf<c64> R64(N), AX(N);
-        DevBuf<c32> R32(N), D32(N);
+        DevBuf<c64> R64(N, w.stream), AX(N, w.stream);
+        DevBuf<c32> R32(N, w.stream), D32(N, w.stream);
         const double bn = norm_fro<double>(w, B64, rows, rows, s);
         fill_zero<double>(w.stream, X64, rows, rows, s);
         BDDA_HIP(hipMemcpyAsync(R64.p, B64, sizeof(c64) * N, hipMemcpyDeviceToDevice, w.stream));

@@ -31,6 +31,8 @@ struct Problem {
     c32* jpre32 = nullptr;
     BlockWork work;
     ~Problem();
+    // device buffers back to the allocator; call before work.stream is destroyed
+    void release();
     int64_t rows() const { return (int64_t)f * nocc; }
     template <typename R>
     const cplx<R>* diag() const;

@@ -200,8 +200,24 @@ int bdda_mbt_set_workspace_limit(bdda_mbt* h, size_t bytes);
  * `bytes` each (from bdda_dev_alloc); their contents are overwritten. */
 int bdda_hbm_probe(void* a, void* b, size_t bytes, double* copy_gbs, double* read_gbs, double* write_gbs);
 
+/* Device memory of the library (no reference counterpart: the reference
+ * keeps its blocks in host memory).  Every device buffer, the library's own
+ * included, comes from one caching allocator per GPU: freed blocks stay with
+ * it and are handed out again (best fit, split, merged with idle neighbours),
+ * because memory new to the process is slow to get from the runtime (single
+ * hipMalloc calls of 0.3-3 s for the tens of GB a large operator needs).
+ * bdda_dev_free waits for the whole device before the block is reused, as
+ * hipFree does.  When the runtime runs out of memory, the wholly idle
+ * allocations go back to it and the request is tried again; bdda_dev_trim
+ * does that on demand (for a caller that shares the GPU with another
+ * allocator) and reports the bytes released.  BDDA_DEVICE_CACHE=0 in the
+ * environment turns the cache off. */
 int bdda_dev_alloc(size_t bytes, void** p);
 int bdda_dev_free(void* p);
+int bdda_dev_trim(size_t* released);
+/* bytes held from the runtime and in live blocks, runtime allocations made
+ * and requests served from idle blocks, on the current device */
+int bdda_dev_pool_stats(size_t* held, size_t* in_use, int64_t* runtime_allocs, int64_t* reuses);
 int bdda_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int bdda_memcpy_d2h(void* dst, const void* src, size_t bytes);
 /* device-to-device copy (also the known-size reference for HBM counters) */
