@@ -41,18 +41,18 @@ void BlockWork::reserve(size_t partial_n, size_t red_n)
         dev_free(partial, stream);
         partial = nullptr;
         partial_elems = 0;
-        partial = dev_alloc_n<c64>(partial_n);
+        partial = dev_alloc_n<c64>(partial_n, true);
         partial_elems = partial_n;
     }
     if (red_n > red_elems) {
         dev_free(red, stream);
         red = nullptr;
         red_elems = 0;
-        red = dev_alloc_n<c64>(red_n);
+        red = dev_alloc_n<c64>(red_n, true);
         red_elems = red_n;
     }
     if (!coef) {
-        coef = dev_alloc_n<c64>((size_t)COEF_SLOTS * COEF_SLOT_ELEMS);
+        coef = dev_alloc_n<c64>((size_t)COEF_SLOTS * COEF_SLOT_ELEMS, true);
         BDDA_HIP(hipHostMalloc(&host_pinned, sizeof(c64) * (COEF_SLOTS * COEF_SLOT_ELEMS + 64 * 64),
                                hipHostMallocDefault));
         host_elems = COEF_SLOTS * COEF_SLOT_ELEMS;

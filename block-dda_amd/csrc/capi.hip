@@ -129,6 +129,16 @@ static void from_dev(const cplx<R>* src, int64_t rows, int cols, bdda_c64* dst, 
         for (int64_t i = 0; i < rows; ++i) dst[j * ld + i] = {(double)tmp[(size_t)j * rows + i].x, (double)tmp[(size_t)j * rows + i].y};
 }
 
+// the pass workspace lives for one call of the C ABI: back to the device
+// allocator when the call returns, so that idle operators hold no workspace
+struct WorkspaceScope {
+    MbtOperator* op;
+    ~WorkspaceScope()
+    {
+        if (op) mbt_release_workspace(op);
+    }
+};
+
 template <typename R>
 static void apply_host(bdda_mbt* h, const bdda_c64* x, int64_t ldx, bdda_c64* y, int64_t ldy, int ncols, int occupied,
                        int jacobi)
@@ -354,6 +364,7 @@ int bdda_mbt_apply(bdda_mbt* h, int prec, const bdda_c64* x, int64_t ldx, bdda_c
     const int64_t rows = (int64_t)h->f * h->n[0] * h->n[1] * h->n[2];
     if (ldx < rows || ldy < rows) return fail(BDDA_EINVAL, "leading dimension smaller than f*nx*ny*nz");
     BDDA_TRY({
+        WorkspaceScope wss{h->op};
         if (prec == BDDA_PREC_FP64) apply_host<double>(h, x, ldx, y, ldy, ncols, 0, 0);
         else apply_host<float>(h, x, ldx, y, ldy, ncols, 0, 0);
         return BDDA_OK;
@@ -392,6 +403,7 @@ int bdda_mbt_apply_occupied(bdda_mbt* h, int prec, int jacobi, const bdda_c64* X
     if (int rc = check_prec(h, prec)) return rc;
     if (ldx < h->prob.rows() || ldy < h->prob.rows()) return fail(BDDA_EINVAL, "leading dimension smaller than f*nocc");
     BDDA_TRY({
+        WorkspaceScope wss{h->op};
         if (prec == BDDA_PREC_FP64) apply_host<double>(h, X, ldx, Y, ldy, ncols, 1, jacobi);
         else apply_host<float>(h, X, ldx, Y, ldy, ncols, 1, jacobi);
         return BDDA_OK;
@@ -407,6 +419,7 @@ int bdda_solve(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64* B,
     if (ldb < rows || ldx < rows) return fail(BDDA_EINVAL, "leading dimension smaller than f*nocc");
     if (L < 1 || L > 64) return fail(BDDA_EINVAL, "number of right-hand sides must be in 1..64");
     BDDA_TRY({
+        WorkspaceScope wss{h->op};
         DevMem<c64> bB((size_t)rows * L, h->stream), bX((size_t)rows * L, h->stream);
         c64* dB = bB.p;
         c64* dX = bX.p;
@@ -422,6 +435,7 @@ int bdda_solve_dev(bdda_mbt* h, int method, int jacobi, int mode, const bdda_c64
 {
     if (!h || !dB || !dX) return fail(BDDA_EINVAL, "null pointer");
     BDDA_TRY({
+        WorkspaceScope wss{h->op};
         return solve_common(h, method, jacobi, mode, reinterpret_cast<const c64*>(dB), L, tol, itermax,
                             reinterpret_cast<c64*>(dX), info, hist, hist_len);
     })
@@ -440,6 +454,7 @@ int bdda_residual_columns(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdd
 {
     if (!h || !B || !X) return fail(BDDA_EINVAL, "null pointer");
     BDDA_TRY({
+        WorkspaceScope wss{h->op};
         if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
         const int64_t rows = h->prob.rows();
         if (L < 1 || ldb < rows || ldx < rows) return fail(BDDA_EINVAL, "bad block shape");
@@ -454,7 +469,10 @@ int bdda_residual_columns(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdd
 int bdda_residual_columns_dev(bdda_mbt* h, const bdda_c64* dB, const bdda_c64* dX, int L, double* res)
 {
     if (!h || !dB || !dX) return fail(BDDA_EINVAL, "null pointer");
-    BDDA_TRY({ return rescol_dev(h, reinterpret_cast<const c64*>(dB), reinterpret_cast<const c64*>(dX), L, res); })
+    BDDA_TRY({
+        WorkspaceScope wss{h->op};
+        return rescol_dev(h, reinterpret_cast<const c64*>(dB), reinterpret_cast<const c64*>(dX), L, res);
+    })
 }
 
 int bdda_cross_sections(bdda_mbt* h, const bdda_c64* B, int64_t ldb, const bdda_c64* X, int64_t ldx, int L,
@@ -632,7 +650,7 @@ int bdda_dev_alloc(size_t bytes, void** p)
     if (!p) return fail(BDDA_EINVAL, "null pointer");
     *p = nullptr;
     BDDA_TRY({
-        *p = dev_malloc(bytes);
+        *p = dev_malloc(bytes, true);
         return BDDA_OK;
     })
 }
@@ -691,6 +709,7 @@ int bdda_mbt_apply_occupied_dev(bdda_mbt* h, int prec, int jacobi, const void* d
     if (!h->have_target) return fail(BDDA_EINVAL, "bdda_mbt_set_target not called");
     if (int rc = check_prec(h, prec)) return rc;
     BDDA_TRY({
+        WorkspaceScope wss{h->op};
         const int64_t rows = h->prob.rows();
         if (prec == BDDA_PREC_FP64)
             mbt_apply_occupied<double>(h->op, (const c64*)dX, rows, (c64*)dY, rows, ncols, h->prob.diag64,

@@ -59,7 +59,7 @@ CachingPool<HipRaw>* current_pool()
 
 }  // namespace
 
-void* dev_malloc(size_t bytes)
+void* dev_malloc(size_t bytes, bool keep)
 {
     if (bytes == 0) return nullptr;
     CachingPool<HipRaw>* pl = pools().on ? current_pool() : nullptr;
@@ -69,7 +69,7 @@ void* dev_malloc(size_t bytes)
         return p;
     }
     int err = 0;
-    void* p = pl->alloc(bytes, &err);
+    void* p = pl->alloc(bytes, keep, &err);
     if (!p) throw HipError{(hipError_t)err, "hipMalloc (device cache)"};
     return p;
 }

@@ -19,8 +19,11 @@
 namespace bdda {
 
 // throws HipError (hipErrorOutOfMemory when nothing fits, after the pool's
-// idle runtime allocations went back to the runtime); nullptr for 0 bytes
-void* dev_malloc(size_t bytes);
+// idle runtime allocations went back to the runtime); nullptr for 0 bytes.
+// keep: the block outlives the call that asks for it (operator data, the
+// caller's buffers); otherwise it is freed before that call returns, and
+// large ones may share runtime allocations with other such blocks.
+void* dev_malloc(size_t bytes, bool keep = false);
 // p: nullptr, or a pointer from dev_malloc (others go to hipFree)
 void dev_free(void* p, hipStream_t st) noexcept;
 // the current device's wholly idle runtime allocations back to the runtime;
@@ -35,12 +38,13 @@ struct DevStats {
 DevStats dev_stats() noexcept;
 
 template <typename T>
-T* dev_alloc_n(size_t n)
+T* dev_alloc_n(size_t n, bool keep = false)
 {
-    return static_cast<T*>(dev_malloc(sizeof(T) * n));
+    return static_cast<T*>(dev_malloc(sizeof(T) * n, keep));
 }
 
-// an owned device array of n T, freed after its stream's work
+// an owned device array of n T for the duration of one call, freed after its
+// stream's work
 template <typename T>
 struct DevPtr {
     T* p = nullptr;

@@ -184,7 +184,7 @@ static void upload_twiddles(int P, void** dst)
         const long double a = -2.0L * 3.14159265358979323846264338327950288L * q / P;
         h[q] = mk((R)std::cos(a), (R)std::sin(a));
     }
-    *dst = dev_alloc_n<cplx<R>>(P);
+    *dst = dev_alloc_n<cplx<R>>(P, true);
     BDDA_HIP(hipMemcpy(*dst, h.data(), sizeof(cplx<R>) * P, hipMemcpyHostToDevice));
 }
 
@@ -239,8 +239,8 @@ MbtOperator* mbt_create(const int n[3], int f, double lf, double kre, double kim
         launch_lines<double, false>(full, tmp, d.Px, d.Px, (int64_t)d.Py * d.Pz * nc, (int64_t)d.Py * d.Pz * nc, 1,
                                     op->ax[0], tw[0], stream);
         op->gh_elems = (size_t)(d.Px / 2 + 1) * (d.Py / 2 + 1) * (d.Pz / 2 + 1);
-        if (want64) op->gh64 = dev_alloc_n<c64>(op->gh_elems * nc);
-        if (want32) op->gh32 = dev_alloc_n<c32>(op->gh_elems * nc);
+        if (want64) op->gh64 = dev_alloc_n<c64>(op->gh_elems * nc, true);
+        if (want32) op->gh32 = dev_alloc_n<c32>(op->gh_elems * nc, true);
         hipLaunchKernelGGL(k_reduce_gh, dim3(grid_for((int64_t)op->gh_elems * nc)), dim3(256), 0, stream, tmp,
                            (c64*)op->gh64, (c32*)op->gh32, d, nc);
         BDDA_HIP(hipGetLastError());
@@ -280,14 +280,14 @@ void mbt_set_address(MbtOperator* op, const int* address_host, int64_t nocc)
 {
     const int64_t C = (int64_t)op->d.nx * op->d.ny * op->d.nz;
     if (!op->c2o) {
-        op->c2o = dev_alloc_n<int>(C);
+        op->c2o = dev_alloc_n<int>(C, true);
         op->c2o_cells = C;
     }
     if (op->addr_n < nocc) {
         dev_free(op->addr_dev, op->stream);
         op->addr_dev = nullptr;
         op->addr_n = 0;
-        op->addr_dev = dev_alloc_n<int>(std::max<int64_t>(nocc, 1));
+        op->addr_dev = dev_alloc_n<int>(std::max<int64_t>(nocc, 1), true);
         op->addr_n = nocc;
     }
     BDDA_HIP(hipMemcpyAsync(op->addr_dev, address_host, sizeof(int) * nocc, hipMemcpyHostToDevice, op->stream));
@@ -302,11 +302,11 @@ void mbt_set_address(MbtOperator* op, const int* address_host, int64_t nocc)
     std::vector<int> ids;
     for (int64_t l = 0; l < nxny; ++l)
         if (mask[l]) ids.push_back((int)l);
-    if (!op->lmask) op->lmask = dev_alloc_n<unsigned char>(nxny);
+    if (!op->lmask) op->lmask = dev_alloc_n<unsigned char>(nxny, true);
     dev_free(op->lines, op->stream);
     op->lines = nullptr;
     op->nlines = 0;
-    op->lines = dev_alloc_n<int>(std::max<size_t>(ids.size(), 1));
+    op->lines = dev_alloc_n<int>(std::max<size_t>(ids.size(), 1), true);
     BDDA_HIP(hipMemcpyAsync(op->lmask, mask.data(), nxny, hipMemcpyHostToDevice, op->stream));
     if (!ids.empty())
         BDDA_HIP(hipMemcpyAsync(op->lines, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, op->stream));
@@ -325,6 +325,13 @@ void mbt_ensure_workspace(MbtOperator* op, size_t bytes)
     }
     op->ws = dev_malloc(bytes);
     op->ws_bytes = bytes;
+}
+
+void mbt_release_workspace(MbtOperator* op)
+{
+    dev_free(op->ws, op->stream);
+    op->ws = nullptr;
+    op->ws_bytes = 0;
 }
 
 template <typename R>

@@ -97,6 +97,9 @@ void mbt_apply_occupied(MbtOperator* op, const cplx<R>* X, int64_t ldx, cplx<R>*
 
 // grow the pass workspace to at least `bytes`
 void mbt_ensure_workspace(MbtOperator* op, size_t bytes);
+// give the pass workspace back to the device allocator (after the stream's
+// work); the C ABI's calls that run block matvecs do so before they return
+void mbt_release_workspace(MbtOperator* op);
 
 // one precision's pass sequence (mbt_f32.hip / mbt_f64.hip); c2o == null
 // selects the cuboid layout

@@ -76,10 +76,10 @@ void problem_set_diag(Problem& P, const c64* diag_host)
     for (void* p : {(void*)P.diag64, (void*)P.jpre64, (void*)P.diag32, (void*)P.jpre32}) dev_free(p, P.work.stream);
     P.diag64 = P.jpre64 = nullptr;
     P.diag32 = P.jpre32 = nullptr;
-    P.diag64 = dev_alloc_n<c64>(n);
-    P.jpre64 = dev_alloc_n<c64>(n);
-    P.diag32 = dev_alloc_n<c32>(n);
-    P.jpre32 = dev_alloc_n<c32>(n);
+    P.diag64 = dev_alloc_n<c64>(n, true);
+    P.jpre64 = dev_alloc_n<c64>(n, true);
+    P.diag32 = dev_alloc_n<c32>(n, true);
+    P.jpre32 = dev_alloc_n<c32>(n, true);
     BDDA_HIP(hipMemcpy(P.diag64, diag_host, sizeof(c64) * n, hipMemcpyHostToDevice));
     BDDA_HIP(hipMemcpy(P.jpre64, jp.data(), sizeof(c64) * n, hipMemcpyHostToDevice));
     BDDA_HIP(hipMemcpy(P.diag32, d32.data(), sizeof(c32) * n, hipMemcpyHostToDevice));

@@ -186,7 +186,6 @@ class _Entry:
         self.key = key
         self.handle = handle
         self.base = base           # operator + target bytes
-        self.ws = 0                # pass workspace high-water mark (kept by the handle)
         self.busy = False
         self.last_used = time.perf_counter()
 
@@ -194,9 +193,10 @@ class _Entry:
 class Server:
     """Dispatcher + workers over a backend; see the module docstring.
 
-    Memory model (bytes): every cached operator holds base + ws; a running
-    batch reserves its Krylov blocks, any workspace growth and, while its
-    operator is being built, the build transient.  A batch starts only when
+    Memory model (bytes): every cached operator holds base; a running batch
+    reserves its pass workspace and Krylov blocks (the library gives both
+    back when the solve returns) and, while its operator is being built, the
+    build transient.  A batch starts only when
     the cached operators plus all reservations stay within hbm_budget after
     evicting idle operators (least recently used first).  A scene that has
     waited longer than max_wait_s blocks admissions behind it until it fits.
@@ -266,7 +266,7 @@ class Server:
             self._cache.clear()
 
     def used_bytes(self):
-        return sum(e.base + e.ws for e in self._cache.values()) + self._reserved
+        return sum(e.base for e in self._cache.values()) + self._reserved
 
     def stats(self):
         ok = [r for r in self.completed if r.error is None]
@@ -292,6 +292,7 @@ class Server:
             "admitted_while_busy": self._admitted_while_busy(ok),
             "refused": sum(1 for r in self.completed if r.error is not None and "budget" in r.error),
             "peak_bytes": self.peak_bytes, "budget_bytes": self.budget, "evictions": self.evictions,
+            "errors": sorted({r.error[:200] for r in self.completed if r.error is not None})[:3],
         }
 
     def _admitted_while_busy(self, reqs):
@@ -359,17 +360,12 @@ class Server:
             fresh = max(init_peak, base + ws + kry)   # need when the operator is built for this batch
             e = self._cache.get(key)
             if e is not None:
-                need = kry + max(0, ws - e.ws)
+                # (the cached operator holds base; the batch needs its
+                # workspace and Krylov blocks, never more than a fresh build)
+                need = ws + kry
                 self._cache.move_to_end(key)
                 if self._evict_for(need, keep=key):
                     return key, group, need, est
-                if self._reserved == 0 and fresh <= self.budget:
-                    # only this scene's idle operator is left and the batch
-                    # does not fit beside it: drop it and build it again
-                    # within the batch's own reservation
-                    self._drop(key)
-                    if self._evict_for(fresh):
-                        return key, group, fresh, est
             else:
                 need = fresh
                 if self._evict_for(need):
@@ -462,7 +458,6 @@ class Server:
                 if e is not None:
                     e.busy = False
                     e.last_used = t1
-                    e.ws = max(e.ws, ws)
                 for r in group:
                     r.finish = t1
                     self.completed.append(r)

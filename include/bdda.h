@@ -181,9 +181,10 @@ int bdda_scattering_matrix(bdda_mbt* h, const bdda_c64* X, int64_t ldx, int npai
  * right-hand sides in `mode`: init_peak while bdda_mbt_create runs (the full
  * kernel grid is transformed in FP64 before its octant is kept), resident
  * bytes of the operator with its target, the pass workspace of an L-column
- * block matvec (allocated at the first solve and kept by the handle; see
- * bdda_mbt_set_workspace_limit) and the Krylov blocks of one solve (freed
- * when it returns). */
+ * block matvec (see bdda_mbt_set_workspace_limit) and the Krylov blocks of
+ * one solve.  The workspace and the Krylov blocks live for one call: every
+ * call that runs block matvecs gives them back to the device allocator
+ * before it returns. */
 int bdda_estimate_bytes(const int n[3], int f, int64_t nocc, int L, int mode, int64_t* init_peak,
                         int64_t* resident, int64_t* workspace, int64_t* krylov);
 /* Cap the pass workspace of a handle: block matvecs whose workspace would

@@ -32,9 +32,8 @@ class FakeBackend:
         self.live = 0
         self.max_live = 0
         # bytes this backend actually holds, as a device would: the build
-        # transient while an operator is created, then its base, the
-        # workspace high-water mark each handle keeps, and the Krylov blocks
-        # of running solves
+        # transient while an operator is created, then its base, and the
+        # workspace and Krylov blocks of running solves
         self.held = 0
         self.max_held = 0
 
@@ -54,26 +53,23 @@ class FakeBackend:
         self._hold(base - init_peak)
         with self.lock:
             self.created.append(scene.key)
-        return {"scene": scene, "base": base, "ws": 0}
+        return {"scene": scene, "base": base}
 
     def release(self, h):
-        self._hold(-(h["base"] + h["ws"]))
+        self._hold(-h["base"])
         with self.lock:
             self.released.append(h["scene"].key)
 
     def solve(self, h, B, tol):
         _, _, ws, kry = self.estimate(h["scene"], B.shape[1])
-        if ws > h["ws"]:
-            self._hold(ws - h["ws"])
-            h["ws"] = ws
-        self._hold(kry)
+        self._hold(ws + kry)
         with self.lock:
             self.live += 1
             self.max_live = max(self.max_live, self.live)
         time.sleep(self.solve_s)
         with self.lock:
             self.live -= 1
-        self._hold(-kry)
+        self._hold(-(ws + kry))
         return 2.0 * B, {"iters": 1, "converged": 1}
 
     def observables(self, h, B, X):
@@ -134,8 +130,9 @@ def test_budget_and_lru_eviction(S, T):
 def test_cached_scene_with_a_larger_request(S, T):
     """A scene's operator is cached from a small request; a wider request of
     the same scene does not fit beside it.  Its operator must not be evicted
-    under a need that assumed it stays: the server either rebuilds it within
-    a fresh reservation (when that fits) or refuses the request."""
+    under a need that assumed it stays: the server refuses the request.  A
+    cached operator then serves requests within its own base plus the
+    batch's workspace and Krylov blocks."""
     sc = scenes(S, T, 1)[0]               # base 100, build transient 200
     # budget 250: the 20-column request needs 100 + 100 + 60 = 260 fresh -> refused
     be = FakeBackend(solve_s=0.005)
@@ -149,16 +146,15 @@ def test_cached_scene_with_a_larger_request(S, T):
     assert wide.error is not None and "budget" in wide.error
     assert be.max_held <= 250, be.max_held
     assert be.created.count(sc.key) == 1
-    # the cached handle keeps the 20-column workspace (100); when the budget
-    # then shrinks to 205, a 2-column request does not fit beside it
-    # (200 + 6) but a fresh build does (max(200, 100 + 10 + 6)): the idle
-    # operator is dropped and built again inside the batch's reservation
+    # a handle keeps no workspace between solves: after a 20-column request
+    # (workspace 100), a budget shrunk to 120 still serves a 2-column request
+    # beside the cached operator (100 + 10 + 6), without building it again
     be = FakeBackend(solve_s=0.005)
     sv = S.Server(be, hbm_budget=300, workers=1).start()
     wide = sv.submit(req(S, sc, 20, 0))
     assert sv.drain(timeout=10)
     with sv._cv:
-        sv.budget = 205
+        sv.budget = 120
     small = sv.submit(req(S, sc, 2, 1))
     assert sv.drain(timeout=10)
     st = sv.stats()
@@ -166,7 +162,7 @@ def test_cached_scene_with_a_larger_request(S, T):
     assert small.error is None and wide.error is None
     assert np.array_equal(small.X, 2.0 * small.B)
     assert be.max_held <= 300, be.max_held
-    assert be.created.count(sc.key) == 2 and st["evictions"] == 1
+    assert be.created.count(sc.key) == 1 and st["evictions"] == 0
     assert be.held == 0
 
 
