@@ -553,7 +553,7 @@ int bdda_estimate_bytes(const int n[3], int f, int64_t nocc, int L, int mode, in
         return fail(BDDA_EUNSUPPORTED, "padded grid beyond the passes' limit (P * f <= 3072 on every axis)");
     BDDA_TRY({
         MbtDims d{n[0], n[1], n[2], f, P[0], P[1], P[2]};
-        const int Pzs = (int)ceil_div(d.Pz, 16) * 16;
+        const int Pzs = mbt_z_pitch(d.Pz);
         const int64_t nc = (f == 3) ? 6 : 9;
         const int64_t G = (int64_t)d.Px * d.Py * d.Pz;
         const int64_t gh = (int64_t)(d.Px / 2 + 1) * (d.Py / 2 + 1) * (d.Pz / 2 + 1) * nc * (16 + 8);

@@ -18,6 +18,16 @@ struct HipRaw {
         return (int)e;
     }
     void release(void* p) { (void)hipFree(p); }
+    // under 4 GiB free on the device (the runtime's own allocations need it)
+    bool low()
+    {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        return fr < (size_t(4) << 30);
+    }
 };
 
 constexpr int kMaxDev = 64;

@@ -17,14 +17,19 @@
 //
 // When the raw allocator reports out of memory, wholly idle raw allocations
 // go back to it one at a time, least recently freed first, and the request is
-// tried again after each.
+// tried again after each.  After every raw allocation, idle ones also go back
+// while the raw allocator reports memory low: the runtime allocates memory of
+// its own too (scratch for kernels that spill registers, at their first
+// dispatch on a queue), and a pool that held every free byte idle would make
+// that fail.
 //
 // Header-only and free of HIP types: the library instantiates it over
 // hipMalloc/hipFree (devmem.hip), and a host test over malloc/free.
 //
 // RAW provides  int alloc(size_t bytes, void** p)  (0 on success, RAW::kOom
-// when out of memory, another nonzero code on any other failure) and
-// void release(void* p).
+// when out of memory, another nonzero code on any other failure),
+// void release(void* p) and bool low() (free memory below the headroom the
+// runtime needs).
 #pragma once
 #include <algorithm>
 #include <cstddef>
@@ -94,12 +99,16 @@ public:
             *err = e;
             return nullptr;
         }
-        std::lock_guard<std::mutex> g(m_);
-        Block* b = new Block{static_cast<char*>(p), raw_bytes, false, k, nullptr, nullptr, 0};
-        st_.held += raw_bytes;
-        ++st_.raw_allocs;
-        use(b, sz);
-        return b->ptr;
+        {
+            std::lock_guard<std::mutex> g(m_);
+            Block* b = new Block{static_cast<char*>(p), raw_bytes, false, k, nullptr, nullptr, 0};
+            st_.held += raw_bytes;
+            ++st_.raw_allocs;
+            use(b, sz);
+        }
+        while (raw_.low() && release_idle(1) > 0) {
+        }
+        return p;
     }
 
     // false when p did not come from this pool (nothing is changed)

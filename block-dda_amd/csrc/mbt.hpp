@@ -49,7 +49,7 @@ struct MbtOperator {
     double lf;
     double kre, kim;
     AxisPlan ax[3];             // x plan ends with radix 8
-    int Pzs = 0;                // row pitch of U and V: Pz rounded up to 16
+    int Pzs = 0;                // row pitch of U and V (mbt_z_pitch): Pz rounded up to 16, padded
     void* tw64[3] = {nullptr, nullptr, nullptr};  // per-axis twiddle tables
     void* tw32[3] = {nullptr, nullptr, nullptr};
     void* gh64 = nullptr;       // reduced G^, FP64 (scaled by 1/(PxPyPz))
@@ -114,6 +114,8 @@ extern template void mbt_apply_precision<double>(MbtOperator*, const cplx<double
 
 // pass workspace of one column (U and V grids) in precision R
 size_t mbt_workspace_per_column(const MbtDims& d, int Pzs, size_t elem_bytes);
+// row pitch Pzs of U and V for a padded z length Pz
+int mbt_z_pitch(int Pz);
 
 // bytes per block matvec under the minimal fused pipeline (the algorithmic
 // model used for roofline numbers)

@@ -341,14 +341,22 @@ __global__ __launch_bounds__(NT) void k_lines_plan(const cplx<R>* __restrict__ i
 }
 
 // ---------------------------------------------------------------- x pass
-// Minimum waves per SIMD asked of the compiler.  FP32 plans whose butterfly
-// registers take at most 42 VGPRs (343 = 7^3: one butterfly of 7 values x 3
-// components per thread) fit in 72 VGPRs, so seven waves fit per SIMD (four
-// groups of seven waves per CU instead of three); larger plans would spill.
+// Minimum waves per SIMD asked of the compiler.
+// - FP32 plans whose butterfly registers take at most 42 VGPRs (343 = 7^3:
+//   one butterfly of 7 values x 3 components per thread) fit in 72 VGPRs, so
+//   seven waves fit per SIMD (four groups of seven waves per CU instead of
+//   three); larger plans would spill.
+// - FP32 plans of one radix-8 butterfly (48 VGPRs of values) took 138-148
+//   VGPRs left alone (336 = 8 7 6, 384 = 8 8 6, 512 = 8 8 7): three waves
+//   per SIMD, one 8-wave group per CU.  Four waves (128 VGPRs, a few spilled
+//   registers) give two groups: x pass 10.6 -> 8.0 ms at 336^3, 15.8 -> 13.3
+//   ms at 384^3, 24.9 -> 23.5 ms at 448^3 (s = 16); five or six waves spill
+//   tens of registers.
 template <typename R, class PL, int F, int NT, int NQ>
 constexpr int xfused_min_waves()
 {
-    return (sizeof(R) == 4 && PlanFFT<R, PL, NT, NQ, F, 1>::E * F * 2 <= 42) ? 7 : 1;
+    constexpr int regs = PlanFFT<R, PL, NT, NQ, F, 1>::E * F * 2;
+    return sizeof(R) != 4 ? 1 : regs <= 42 ? 7 : regs <= 48 ? 4 : 1;
 }
 
 // A group owns (rhs, ky, NQ consecutive kz) across all F component planes.

@@ -39,6 +39,8 @@ struct MallocRaw {
         sizes->erase(p);
         std::free(p);
     }
+    size_t headroom = 0;
+    bool low() { return used->load() + headroom > cap; }
 };
 
 int failures = 0;
@@ -54,7 +56,7 @@ struct Env {
     std::atomic<size_t> used{0};
     std::map<void*, size_t> sizes;
     std::mutex m;
-    MallocRaw raw(size_t cap) { return MallocRaw{&used, cap, &sizes, &m}; }
+    MallocRaw raw(size_t cap, size_t headroom = 0) { return MallocRaw{&used, cap, &sizes, &m, headroom}; }
 };
 
 constexpr size_t MiB = 1 << 20;
@@ -155,6 +157,24 @@ void out_of_memory()
     CHECK(pool.trim() == 0 && pool.stats().held == 128 * MiB);
     CHECK(pool.free(e));
     CHECK(pool.trim() == 128 * MiB && env.used.load() == 0);
+}
+
+// after a raw allocation leaves less than the headroom free, idle raw
+// allocations go back, least recently freed first
+void headroom()
+{
+    Env env;
+    bdda::CachingPool<MallocRaw> pool(env.raw(600 * MiB, 128 * MiB));
+    int err = 0;
+    void* a = pool.alloc(200 * MiB, true, &err);
+    void* b = pool.alloc(100 * MiB, true, &err);
+    CHECK(a && b && pool.free(a) && pool.free(b));
+    void* c = pool.alloc(250 * MiB, true, &err);  // fits, but 550 MiB held is over 600 - 128
+    CHECK(c && pool.stats().raw_releases == 1 && pool.stats().held == 350 * MiB);
+    auto idle = pool.idle_blocks();
+    CHECK(idle.size() == 1 && idle[0].first == b);
+    CHECK(pool.free(c));
+    CHECK(pool.trim() == 350 * MiB);
 }
 
 // a block's first and last 4 KiB and one byte in every 64 KiB between
@@ -275,6 +295,7 @@ int main()
 {
     basic();
     out_of_memory();
+    headroom();
     random_sequence(1, 768 * MiB, 6000);
     random_sequence(2, 320 * MiB, 6000);   // tight: out-of-memory and trimming on the way
     random_sequence(3, 4096 * MiB, 4000);
