@@ -8,4 +8,6 @@ namespace bdda {
 #define BDDA_PLAN_DEF(p, ...) BDDA_PLAN_INST(, float, PlanOf##p)
 BDDA_PLAN_TABLE_2(BDDA_PLAN_DEF)
 #undef BDDA_PLAN_DEF
+BDDA_PLAN_INST_X(, float, PlanX384F32, 3)
+BDDA_PLAN_INST_X(, float, PlanX384F32, 6)
 }  // namespace bdda

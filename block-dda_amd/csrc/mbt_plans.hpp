@@ -88,6 +88,10 @@ inline bool with_xplan(int P, F&& f)
             f(Radices<8, 8, 3>{});
             return true;
         }
+        if (P == 384) {  // (8 8 6: 12.8 ms at 384^3, s = 16; 6 4 4 4 on 384 threads: 11.2 ms)
+            f(Radices<6, 4, 4, 4>{});
+            return true;
+        }
     }
     return with_plan(P, f);
 }
@@ -627,6 +631,17 @@ void lines_plan_launch(const cplx<R>* in, cplx<R>* out, int nlin, int nlout, int
     BDDA_HIP(hipGetLastError());
 }
 
+// threads of an x group for a plan (0: plan_threads' choice, one round of
+// butterflies per stage where that fills the lanes best).  Measured FP32 x
+// pass, s = 16, at P^3: two butterflies per thread on 256 threads at 280
+// (8 7 5: 4.10 -> 3.70 ms) and 336 (8 7 6: 8.06 -> 6.33 ms), against 8.06
+// ms for 336 on 448 threads; at 320 (8 8 5: 5.48 -> 6.64 ms), 448 (8 8 7:
+// 21.0 -> 29.0 ms) and 512 (8 8 8: 27.5 -> 30.1 ms) the same was slower.
+template <typename R, class PL>
+struct XThreads {
+    static constexpr int value = 0;
+};
+
 // false: the grid's lane offsets do not fit 32 bits (the all-in-LDS kernel takes it)
 template <typename R, class PL, int F>
 bool xfused_plan_launch(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, int Pzs, const cplx<R>* tw, int s,
@@ -646,7 +661,7 @@ bool xfused_plan_launch(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, int Pzs
         constexpr int NQ = T::NQ_X;
         const int ntz = (int)ceil_div(d.Pz, NQ);
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * NQ);
-        constexpr int NT = plan_threads<PL, NQ, 2, 512>();
+        constexpr int NT = XThreads<R, PL>::value ? XThreads<R, PL>::value : plan_threads<PL, NQ, 2, 512>();
         hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, NQ>), dim3((unsigned)((int64_t)s * d.Py * ((ntz + 1) & ~1))),
                            dim3(NT), shm, st, V, gh, d, Pzs, tw, ntz, s);
         BDDA_HIP(hipGetLastError());
@@ -793,7 +808,20 @@ inline bool launch_zinv_plan(const cplx<R>* U, const int* c2o, const cplx<R>* X,
 #define BDDA_PLAN_TYPE(p, ...) using PlanOf##p = Radices<__VA_ARGS__>;
 BDDA_PLAN_TABLE(BDDA_PLAN_TYPE)
 #undef BDDA_PLAN_TYPE
-using PlanX192F32 = Radices<8, 8, 3>;  // with_xplan's FP32 x plan at 192
+using PlanX192F32 = Radices<8, 8, 3>;  // with_xplan's FP32 x plans at 192 and 384
+using PlanX384F32 = Radices<6, 4, 4, 4>;
+template <>
+struct XThreads<float, PlanX384F32> {
+    static constexpr int value = 384;
+};
+template <>
+struct XThreads<float, PlanOf336> {
+    static constexpr int value = 256;
+};
+template <>
+struct XThreads<float, PlanOf280> {
+    static constexpr int value = 256;
+};
 
 #ifndef BDDA_PLAN_INSTANTIATE
 #define BDDA_PLAN_EXTERN(p, ...) BDDA_PLAN_INST(extern, float, PlanOf##p) BDDA_PLAN_INST(extern, double, PlanOf##p)
@@ -801,6 +829,8 @@ BDDA_PLAN_TABLE(BDDA_PLAN_EXTERN)
 #undef BDDA_PLAN_EXTERN
 BDDA_PLAN_INST_X(extern, float, PlanX192F32, 3)
 BDDA_PLAN_INST_X(extern, float, PlanX192F32, 6)
+BDDA_PLAN_INST_X(extern, float, PlanX384F32, 3)
+BDDA_PLAN_INST_X(extern, float, PlanX384F32, 6)
 #endif
 
 }  // namespace bdda
