@@ -363,7 +363,21 @@ constexpr int xfused_min_waves()
     return sizeof(R) != 4 ? 1 : regs <= 42 ? 7 : regs <= 48 ? 4 : 1;
 }
 
-// A group owns (rhs, ky, NQ consecutive kz) across all F component planes.
+// Right-hand sides per x group, one after the other.  A group reads the G^
+// rows of its (ky, kz) tile for each right-hand side; at the large planned
+// sizes the right-hand sides' groups of one tile no longer find those rows in
+// L2 (512^3: 41.7 GB fetched against 26.6 GB algorithmic).  Two per group
+// read them while they are still there: x pass at s = 16, FP32 512^3 28.0 ->
+// 20.5 ms, FP64 512^3 56.4 -> 43.4 ms and 448^3 58.7 -> 50.0 ms.  Four were
+// no better; FP32 at 448 spills with two (57 registers); 343 and below gain
+// nothing.
+template <typename R, class PL>
+struct XRhsPerGroup {
+    static constexpr int value = (sizeof(R) == 4 ? PL::P == 512 : PL::P >= 448) ? 2 : 1;
+};
+
+// A group owns (RPG right-hand sides, ky, NQ consecutive kz) across all F
+// component planes.
 template <typename R, class PL, int F, int NT, int NQ>
 __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_xfused_plan(
     cplx<R>* __restrict__ V, const cplx<R>* __restrict__ Gh, MbtDims d, int Pzs, const cplx<R>* __restrict__ twg,
@@ -377,15 +391,18 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
     const int64_t plane = (int64_t)d.Py * Pzs;
     const int64_t cstride = (int64_t)d.nx * plane;
     int64_t tile;
-    int rhs;
-    // kz tiles in pairs (ntz rounded up to even; the padding groups exit)
+    int rg;
+    // kz tiles in pairs (ntz rounded up to even; the padding groups exit); a
+    // group takes RPG right-hand sides one after the other
+    constexpr int RPG = XRhsPerGroup<R, PL>::value;
     const int ntzp = (ntz + 1) & ~1;
-    xcd_tile_rhs<2>(blockIdx.x, (int64_t)d.Py * ntzp, nrhs, tile, rhs);
+    xcd_tile_rhs<2>(blockIdx.x, (int64_t)d.Py * ntzp, (nrhs + RPG - 1) / RPG, tile, rg);
     const int ky = (int)(tile / ntzp);
     const int kz0 = (int)(tile % ntzp) * NQ;
     if (kz0 >= d.Pz) return;  // whole group, before any barrier
     const int tqn = min(NQ, d.Pz - kz0);
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
+    auto one_rhs = [&](const int rhs) {
     cplx<R>* base = V + (int64_t)rhs * F * cstride + (int64_t)ky * Pzs + kz0;
     PlanFFT<R, PL, NT, NQ, F, 1> X(tw, lds);
     // V addresses split into a uniform row base (component, compile-time row
@@ -454,6 +471,13 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
         const int p = j + decltype(RB)::value;
         if (p < d.nx && q < tqn) *vaddr(j, RB, q, c) = v;
     });
+    };
+    if constexpr (RPG == 1) {
+        one_rhs(rg);
+    } else {
+        for (int rr = 0; rr < RPG; ++rr)
+            if (rg * RPG + rr < nrhs) one_rhs(rg * RPG + rr);  // (whole group)
+    }
 }
 
 // ---------------------------------------------------------------- z passes
@@ -662,7 +686,9 @@ bool xfused_plan_launch(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, int Pzs
         const int ntz = (int)ceil_div(d.Pz, NQ);
         const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * NQ);
         constexpr int NT = XThreads<R, PL>::value ? XThreads<R, PL>::value : plan_threads<PL, NQ, 2, 512>();
-        hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, NQ>), dim3((unsigned)((int64_t)s * d.Py * ((ntz + 1) & ~1))),
+        constexpr int RPG = XRhsPerGroup<R, PL>::value;
+        hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, NQ>),
+                           dim3((unsigned)((int64_t)((s + RPG - 1) / RPG) * d.Py * ((ntz + 1) & ~1))),
                            dim3(NT), shm, st, V, gh, d, Pzs, tw, ntz, s);
         BDDA_HIP(hipGetLastError());
         return true;
