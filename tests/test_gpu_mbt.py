@@ -202,6 +202,38 @@ def test_plan_sizes(bdda, orc, P, axis, f):
     assert rel(op.apply_occupied(X, "fp32", True), ref) <= TOL32
 
 
+# sizes whose passes are tuned apart from the table's defaults: FP32 x plans
+# on 256 threads (280, 336) or as 6 4 4 4 (384), x groups over two
+# right-hand sides (FP32 512, FP64 448 and 512), row pitches padded off
+# multiples of 64 elements (384, 448, 512 along z)
+TUNED_CASES = [(P, ax, 3) for P in (280, 320, 336, 384, 448, 512) for ax in (0, 1, 2)]
+TUNED_CASES += [(P, 0, 6) for P in (384, 448, 512)]
+
+
+@pytest.mark.parametrize("P,axis,f", TUNED_CASES)
+def test_tuned_sizes(bdda, orc, P, axis, f):
+    """Occupied matvec on a grid one of whose axes pads to P against the
+    oracle, FP64 and FP32, with Jacobi scaling, over 3 right-hand sides (an
+    odd count, so a group of two right-hand sides is left with one)."""
+    print("case tuned size", P, "axis", axis, "f", f, flush=True)
+    rng = np.random.default_rng(P * 11 + axis * 5 + f)
+    n = [5, 4, 6]
+    n[axis] = P // 2
+    n = tuple(n)
+    C = int(np.prod(n))
+    addr = np.sort(rng.choice(C, size=C // 3, replace=False)).astype(np.int32)
+    diag = np.repeat(1.0 / (rng.standard_normal(addr.size) * 0.1 + 0.3 + 0.05j), f)
+    lf, k = 0.19, 1.3 + 0.02j
+    op = bdda.MBT_fft_init(n, f, lf, k)
+    assert op.grid[axis] == P
+    op.set_target(addr, diag)
+    Au = orc.mbt_fft_init(n, f, lf, k)
+    X = rnd(rng, f * addr.size, 3)
+    ref = orc.block_matvec(n, f, addr, Au, diag, X, jacobi=True)
+    assert rel(op.apply_occupied(X, "fp64", True), ref) <= TOL64
+    assert rel(op.apply_occupied(X, "fp32", True), ref) <= TOL32
+
+
 @pytest.mark.parametrize("jac", [False, True])
 def test_occupied_lines_against_cuboid(bdda, jac):
     """Occupied matvec (only occupied lines transformed along z) against the
