@@ -75,6 +75,8 @@ class Request:
         self.observables = None
         self.error = None
         self.batch_size = 0
+        self.retries = 0           # batches of this request that ran out of device memory and were queued again
+        self.oom = False
         self.done = threading.Event()
 
     @property
@@ -132,6 +134,10 @@ class GpuBackend:
     def release(self, op):
         op.close()
 
+    def trim(self):
+        """Give the device allocator's idle memory back to the runtime."""
+        self.pkg.dev_trim()
+
     def solve(self, op, B, tol):
         X, info, _ = op.solve(B, tol, self.itermax, method=self.method, jacobi=self.jacobi, mode=self.mode)
         return X, info
@@ -181,6 +187,11 @@ class GpuBackend:
             L.bdda_dev_free(dB)
 
 
+def _out_of_memory(e):
+    """The library's ENOMEM status (or a message that says so)."""
+    return getattr(e, "code", None) == 2 or "out of memory" in str(e).lower()
+
+
 class _Entry:
     def __init__(self, key, handle, base):
         self.key = key
@@ -202,9 +213,12 @@ class Server:
     waited longer than max_wait_s blocks admissions behind it until it fits.
     """
 
-    def __init__(self, backend, hbm_budget, workers=4, max_cols=64, merge=True, max_wait_s=2.0, on_batch=None):
+    def __init__(self, backend, hbm_budget, workers=4, max_cols=64, merge=True, max_wait_s=2.0, on_batch=None,
+                 oom_retries=1):
         self.backend = backend
-        self.on_batch = on_batch      # called by the worker with (scene key, requests, seconds) after each batch
+        self.oom_retries = int(oom_retries)   # times a request is queued again after running out of device memory
+        self.requeued = 0
+        self.on_batch = on_batch      # called by the worker with (scene key, finished requests, seconds) after each batch
         self.budget = int(hbm_budget)
         self.max_cols = int(max_cols)
         self.merge = merge
@@ -293,6 +307,7 @@ class Server:
             "refused": sum(1 for r in self.completed if r.error is not None and "budget" in r.error),
             "peak_bytes": self.peak_bytes, "budget_bytes": self.budget, "evictions": self.evictions,
             "errors": sorted({r.error[:200] for r in self.completed if r.error is not None})[:3],
+            "requeued_after_oom": self.requeued,
         }
 
     def _admitted_while_busy(self, reqs):
@@ -450,7 +465,17 @@ class Server:
             except Exception as e:  # reported per request; the server keeps running
                 for r in group:
                     r.error = repr(e)
+                    r.oom = _out_of_memory(e)
             t1 = time.perf_counter()
+            # out of device memory (the estimates are estimates; idle memory
+            # can be held by the allocator): idle memory goes back to the
+            # runtime and the requests queue again, ahead of later arrivals
+            retry = [r for r in group if r.error is not None and r.oom and r.retries < self.oom_retries]
+            if retry and hasattr(self.backend, "trim"):
+                try:
+                    self.backend.trim()
+                except Exception:
+                    pass
             with self._cv:
                 self._reserved -= need
                 self._inflight.discard(key)
@@ -458,15 +483,25 @@ class Server:
                 if e is not None:
                     e.busy = False
                     e.last_used = t1
-                for r in group:
+                done = [r for r in group if r not in retry]
+                for r in retry:
+                    r.retries += 1
+                    r.error = None
+                    r.oom = False
+                    r.start = None
+                if retry:
+                    self.requeued += len(retry)
+                    self._pending = collections.OrderedDict(
+                        [(r.id, r) for r in retry] + list(self._pending.items()))
+                for r in done:
                     r.finish = t1
                     self.completed.append(r)
                 self.batches.append((key, [r.id for r in group], t0, t1, wid))
                 self._cv.notify_all()
-            for r in group:
+            for r in done:
                 r.done.set()
-            if self.on_batch is not None:
-                self.on_batch(key, group, t1 - t0)
+            if self.on_batch is not None and done:
+                self.on_batch(key, done, t1 - t0)
 
 
 # -------------------------------------------------------------------- streams

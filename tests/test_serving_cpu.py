@@ -166,6 +166,63 @@ def test_cached_scene_with_a_larger_request(S, T):
     assert be.held == 0
 
 
+class OomError(RuntimeError):
+    code = 2   # the library's ENOMEM status
+
+
+class OomBackend(FakeBackend):
+    """Runs out of device memory on the first `fails` solves of each scene;
+    counts the trims the server asks for."""
+
+    def __init__(self, fails=1, **kw):
+        super().__init__(**kw)
+        self.fails = fails
+        self.seen = {}
+        self.trims = 0
+
+    def trim(self):
+        with self.lock:
+            self.trims += 1
+
+    def solve(self, h, B, tol):
+        key = h["scene"].key
+        with self.lock:
+            n = self.seen.get(key, 0)
+            self.seen[key] = n + 1
+        if n < self.fails:
+            raise OomError("out of memory")
+        return super().solve(h, B, tol)
+
+
+def test_out_of_memory_batches_queue_again(S, T):
+    """A batch that runs out of device memory gives idle memory back (trim)
+    and its requests queue again, ahead of later ones; a request is retried
+    once, so a second failure is reported."""
+    be = OomBackend(fails=1)
+    sv = S.Server(be, hbm_budget=10 ** 6, workers=2).start()
+    sc = scenes(S, T, 2)
+    reqs = [req(S, sc[i % 2], 2, i) for i in range(6)]
+    for r in reqs:
+        sv.submit(r)
+    assert sv.drain(timeout=30)
+    st = sv.stats()
+    sv.stop()
+    assert st["failed"] == 0 and st["requests"] == 6
+    assert st["requeued_after_oom"] >= 2 and be.trims >= 2
+    assert all(np.array_equal(r.X, 2.0 * r.B) for r in reqs)
+    assert sum(r.retries for r in reqs) == st["requeued_after_oom"]
+    assert all(r.retries <= 1 for r in reqs)
+    # a scene that keeps running out of memory: reported after one retry
+    be = OomBackend(fails=5)
+    sv = S.Server(be, hbm_budget=10 ** 6, workers=1).start()
+    r = sv.submit(req(S, sc[0], 2, 9))
+    assert sv.drain(timeout=30)
+    st = sv.stats()
+    sv.stop()
+    assert r.error is not None and "out of memory" in r.error and r.retries == 1
+    assert st["requests"] == 1 and st["failed"] == 1
+
+
 def test_too_large_request_fails_alone(S, T):
     be = FakeBackend()
     sc = scenes(S, T, 2)
