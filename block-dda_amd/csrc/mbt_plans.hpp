@@ -642,6 +642,16 @@ __global__ __launch_bounds__(NT) void k_zinv_plan(const cplx<R>* __restrict__ U,
 // one per group of the plan table, so the kernels compile in parallel; every
 // other unit sees extern declarations (end of this file) and only dispatches.
 
+// threads of a y group for a plan (0: plan_threads' choice).  FP32 343 (7 7
+// 7, 784 butterflies per stage over 16 columns) gets 448 threads with two
+// butterflies each instead of 832 with one: y forward 2.74 -> 2.52 ms, y
+// inverse 2.35 -> 2.13 ms at the 16384-monomer workload (s = 16); in FP64
+// 256 threads were no faster and 128 slower.
+template <typename R, class PL>
+struct YThreads {
+    static constexpr int value = 0;
+};
+
 template <typename R, class PL, bool INV>
 void lines_plan_launch(const cplx<R>* in, cplx<R>* out, int nlin, int nlout, int64_t Zdim, int64_t pitch,
                        int64_t outer, const cplx<R>* tw, hipStream_t st, const unsigned char* lmask, int nxm)
@@ -649,7 +659,7 @@ void lines_plan_launch(const cplx<R>* in, cplx<R>* out, int nlin, int nlout, int
     using T = PlanTiles<R>;
     const int64_t ntiles = ceil_div(Zdim, T::NQ_Y);
     const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * T::NQ_Y);
-    constexpr int NT = plan_threads<PL, T::NQ_Y, 4, 256>();
+    constexpr int NT = YThreads<R, PL>::value ? YThreads<R, PL>::value : plan_threads<PL, T::NQ_Y, 4, 256>();
     hipLaunchKernelGGL((k_lines_plan<R, PL, INV, NT, T::NQ_Y>), dim3((unsigned)(ntiles * outer)), dim3(NT), shm, st,
                        in, out, nlin, nlout, Zdim, pitch, ntiles, tw, lmask, nxm);
     BDDA_HIP(hipGetLastError());
@@ -847,6 +857,10 @@ struct XThreads<float, PlanOf336> {
 template <>
 struct XThreads<float, PlanOf280> {
     static constexpr int value = 256;
+};
+template <>
+struct YThreads<float, PlanOf343> {
+    static constexpr int value = 448;
 };
 
 #ifndef BDDA_PLAN_INSTANTIATE
