@@ -98,8 +98,12 @@ __device__ __forceinline__ void dft_odd_nz(cplx<R>* v)
 // R: real type, PL: Radices<...>, NT threads, NQ lines (power of two), NC
 // components, NCX components per LDS exchange round.  Methods take INV: the
 // inverse transform uses the radices in reverse order, so the forward's last
-// stage and the inverse's first stage share radix and butterfly layout.
-template <typename R, class PL, int NT, int NQ, int NC, int NCX>
+// stage and the inverse's first stage share radix and butterfly layout.  DB:
+// the exchange uses two LDS buffers of NCX * P * NQ elements (lds points at
+// the first, the second follows it) and one barrier per round.  An instance
+// used for several transforms in a row needs an even number of rounds per
+// forward + inverse pair, which 2 (NS - 1) rounds of each component set are.
+template <typename R, class PL, int NT, int NQ, int NC, int NCX, bool DB = false>
 struct PlanFFT {
     static constexpr int P = PL::P;
     static constexpr int NS = PL::N;
@@ -282,8 +286,15 @@ struct PlanFFT {
         constexpr int RO = rad<S, INV>(), L = Lof<S, INV>(), nbo = P / RO, to = nbo * NQ;
         constexpr int RI = rad<S + 1, INV>(), nbi = P / RI, ti = nbi * NQ;
         constexpr int PQ = P * NQ;
-        static_for<0, (NC + NCX - 1) / NCX>([&](auto CB) {
+        constexpr int NR = (NC + NCX - 1) / NCX;
+        static_for<0, NR>([&](auto CB) {
             constexpr int c0 = decltype(CB)::value * NCX;
+            // DB: rounds alternate between two buffers (round number counted
+            // over the forward's exchanges, then the inverse's), so a round's
+            // stores cannot meet the previous round's loads and one barrier
+            // per round is enough
+            constexpr int rnd = ((INV ? NS - 1 : 0) + S) * NR + decltype(CB)::value;
+            cplx<R>* lds = this->lds + (DB && (rnd & 1) ? NCX * PQ : 0);
             static_for<0, mb<S, INV>()>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 const int b = threadIdx.x + i * NT;
@@ -310,7 +321,7 @@ struct PlanFFT {
                             if (c0 + cc < NC) v[c0 + cc][i * RI + r] = lds[cc * PQ + b + r * nbi * NQ];
                 }
             });
-            __syncthreads();
+            if constexpr (!DB) __syncthreads();
         });
     }
 

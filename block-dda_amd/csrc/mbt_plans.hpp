@@ -376,6 +376,22 @@ struct XRhsPerGroup {
     static constexpr int value = (sizeof(R) == 4 ? PL::P == 512 : PL::P >= 448) ? 2 : 1;
 };
 
+// Two LDS exchange buffers for the FP32 x groups (PlanFFT's DB): one barrier
+// per exchange round instead of two (13 per right-hand side instead of 25 for
+// three stages), for twice the exchange LDS.  Measured x pass at P^3, s = 16,
+// one buffer -> two: 128 0.347 -> 0.327 ms, 160 0.99 -> 0.95, 168 0.83 ->
+// 0.80, 200 1.81 -> 1.53, 224 1.82 -> 1.76, 240 2.90 -> 2.77, 288 4.68 ->
+// 4.61, 320 5.47 -> 5.42, 384 11.02 -> 10.94, 448 20.0 -> 19.1; slower at 192
+// (+2 %), 256 (+1 %), 336 (+4 %) and 343 (+0.5 %), where the larger LDS
+// leaves fewer groups per CU; sizes below 128 within noise.  Plans whose two
+// buffers would pass 64 KB per group keep one.
+template <typename R, class PL>
+struct XDoubleBuffer {
+    static constexpr int P = PL::P;
+    static constexpr bool value = sizeof(R) == 4 && P * (1 + 2 * PlanTiles<R>::NQ_X) * 8 <= 65536 && P != 192 &&
+                                  P != 256 && P != 336 && P != 343;
+};
+
 // A group owns (RPG right-hand sides, ky, NQ consecutive kz) across all F
 // component planes.
 template <typename R, class PL, int F, int NT, int NQ>
@@ -404,7 +420,7 @@ __global__ __launch_bounds__(NT, (xfused_min_waves<R, PL, F, NT, NQ>())) void k_
     for (int i = threadIdx.x; i < P; i += NT) tw[i] = twg[i];
     auto one_rhs = [&](const int rhs) {
     cplx<R>* base = V + (int64_t)rhs * F * cstride + (int64_t)ky * Pzs + kz0;
-    PlanFFT<R, PL, NT, NQ, F, 1> X(tw, lds);
+    PlanFFT<R, PL, NT, NQ, F, 1, XDoubleBuffer<R, PL>::value> X(tw, lds);
     // V addresses split into a uniform row base (component, compile-time row
     // offset RB) and a 32-bit per-lane byte offset (j plane + q, j < P/r0;
     // the launcher checks that it fits), so each access is a scalar-base
@@ -694,7 +710,7 @@ bool xfused_plan_launch(cplx<R>* V, const cplx<R>* gh, const MbtDims& d, int Pzs
         // 14 % slower here: their row segments straddle 64-byte boundaries)
         constexpr int NQ = T::NQ_X;
         const int ntz = (int)ceil_div(d.Pz, NQ);
-        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * NQ);
+        const size_t shm = sizeof(cplx<R>) * (PL::P + (size_t)PL::P * NQ * (XDoubleBuffer<R, PL>::value ? 2 : 1));
         constexpr int NT = XThreads<R, PL>::value ? XThreads<R, PL>::value : plan_threads<PL, NQ, 2, 512>();
         constexpr int RPG = XRhsPerGroup<R, PL>::value;
         hipLaunchKernelGGL((k_xfused_plan<R, PL, F, NT, NQ>),
