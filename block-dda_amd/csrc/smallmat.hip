@@ -50,8 +50,8 @@ __global__ __launch_bounds__(SM_NT) void k_lu_solve(const c64* __restrict__ A, c
     __shared__ c64 a[SM_MAX * SM_MAX];
     __shared__ c64 x[SM_MAX * SM_MAX];
     __shared__ int rp[SM_MAX], cp[SM_MAX];
-    __shared__ double bestv[SM_NT];
-    __shared__ int besti[SM_NT];
+    __shared__ double bestv[SM_NT / 64];
+    __shared__ int besti[SM_NT / 64];
     __shared__ int piv[2];
     __shared__ int fail;
     const int t = threadIdx.x;
@@ -77,25 +77,34 @@ __global__ __launch_bounds__(SM_NT) void k_lu_solve(const c64* __restrict__ A, c
                 bi = key;
             }
         }
-        bestv[t] = bv;
-        besti[t] = bi;
+        // (largest modulus, then smallest key) is a total order, so any
+        // reduction order picks the same pivot: lane shuffles within each
+        // wave, then the waves' winners in LDS (2 barriers per column instead
+        // of a barrier per halving over the group)
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(bv, off);
+            const int oi = __shfl_xor(bi, off);
+            if (o > bv || (o == bv && oi < bi)) {
+                bv = o;
+                bi = oi;
+            }
+        }
+        if ((t & 63) == 0) {
+            bestv[t >> 6] = bv;
+            besti[t >> 6] = bi;
+        }
         __syncthreads();
-        for (int w = SM_NT / 2; w > 0; w >>= 1) {
-            if (t < w) {
-                const double o = bestv[t + w];
-                const int oi = besti[t + w];
-                if (o > bestv[t] || (o == bestv[t] && oi < besti[t])) {
-                    bestv[t] = o;
-                    besti[t] = oi;
+        if (t == 0) {
+            for (int w = 1; w < SM_NT / 64; ++w) {
+                if (bestv[w] > bv || (bestv[w] == bv && besti[w] < bi)) {
+                    bv = bestv[w];
+                    bi = besti[w];
                 }
             }
-            __syncthreads();
-        }
-        if (t == 0) {
             // (no candidate: every modulus NaN; the pivot a_kk then fails below)
-            const bool none = besti[0] == 0x7fffffff;
-            piv[0] = none ? k : besti[0] % s;  // row
-            piv[1] = none ? k : besti[0] / s;  // column
+            const bool none = bi == 0x7fffffff;
+            piv[0] = none ? k : bi % s;  // row
+            piv[1] = none ? k : bi / s;  // column
         }
         __syncthreads();
         const int pr = piv[0], pc = piv[1];
@@ -213,6 +222,7 @@ __global__ __launch_bounds__(SM_NT) void k_chol_inv(const c64* __restrict__ G, c
                                                     int* __restrict__ flag, int bit)
 {
     __shared__ c64 u[SM_MAX * SM_MAX];
+    __shared__ c64 ui[SM_MAX * SM_MAX];  // the inverse, built in LDS and then stored
     __shared__ int fail;
     const int t = threadIdx.x;
     for (int e = t; e < s * s; e += SM_NT) u[e] = mk(0.0, 0.0);
@@ -248,19 +258,23 @@ __global__ __launch_bounds__(SM_NT) void k_chol_inv(const c64* __restrict__ G, c
         return;
     }
     for (int e = t; e < s * s; e += SM_NT) U[e] = u[e];
-    // inverse by back substitution, one thread per column of Ui
+    // inverse by back substitution, one thread per column of Ui; each step
+    // reads the column's earlier entries, so the column is built in LDS
+    // (through global memory every step would wait for a store to come back)
     for (int j = t; j < s; j += SM_NT) {
-        for (int i = j + 1; i < s; ++i) Ui[i + j * s] = mk(0.0, 0.0);
+        for (int i = j + 1; i < s; ++i) ui[i + j * s] = mk(0.0, 0.0);
         for (int i = j; i >= 0; --i) {
             c64 v = (i == j) ? mk(1.0, 0.0) : mk(0.0, 0.0);
             for (int k = i + 1; k <= j; ++k) {
-                const c64 a = u[i + k * s], b = Ui[k + j * s];
+                const c64 a = u[i + k * s], b = ui[k + j * s];
                 v.x -= a.x * b.x - a.y * b.y;
                 v.y -= a.x * b.y + a.y * b.x;
             }
-            Ui[i + j * s] = cdivd(v, u[i + i * s]);
+            ui[i + j * s] = cdivd(v, u[i + i * s]);
         }
     }
+    __syncthreads();
+    for (int e = t; e < s * s; e += SM_NT) Ui[e] = ui[e];
 }
 
 // out = sqrt(sum |A_e|^2) over n entries, in a fixed order
