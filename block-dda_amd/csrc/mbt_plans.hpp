@@ -381,14 +381,15 @@ struct XRhsPerGroup {
 // three stages), for twice the exchange LDS.  Measured x pass at P^3, s = 16,
 // one buffer -> two: 128 0.347 -> 0.327 ms, 160 0.99 -> 0.95, 168 0.83 ->
 // 0.80, 200 1.81 -> 1.53, 224 1.82 -> 1.76, 240 2.90 -> 2.77, 288 4.68 ->
-// 4.61, 320 5.47 -> 5.42, 384 11.02 -> 10.94, 448 20.0 -> 19.1; slower at 192
-// (+2 %), 256 (+1 %), 336 (+4 %) and 343 (+0.5 %), where the larger LDS
-// leaves fewer groups per CU; sizes below 128 within noise.  Plans whose two
-// buffers would pass 64 KB per group keep one.
+// 4.61, 320 5.47 -> 5.42, 384 11.02 -> 10.94, 448 20.0 -> 19.1, 512 21.7 ->
+// 21.3, 576 52.5 -> 49.2; slower at 192 (+2 %), 256 (+1 %), 336 (+4 %) and
+// 343 (+0.5 %), where the larger LDS leaves fewer groups per CU; sizes below
+// 128 within noise.  Plans whose two buffers would pass 80 KB per group (two
+// groups in a CU's 160 KB) keep one.
 template <typename R, class PL>
 struct XDoubleBuffer {
     static constexpr int P = PL::P;
-    static constexpr bool value = sizeof(R) == 4 && P * (1 + 2 * PlanTiles<R>::NQ_X) * 8 <= 65536 && P != 192 &&
+    static constexpr bool value = sizeof(R) == 4 && P * (1 + 2 * PlanTiles<R>::NQ_X) * 8 <= 80 * 1024 && P != 192 &&
                                   P != 256 && P != 336 && P != 343;
 };
 
