@@ -205,7 +205,9 @@ def test_plan_sizes(bdda, orc, P, axis, f):
 # sizes whose passes are tuned apart from the table's defaults: FP32 x plans
 # on 256 threads (280, 336) or as 6 4 4 4 (384), x groups over two
 # right-hand sides (FP32 512, FP64 448 and 512), row pitches padded off
-# multiples of 64 elements (384, 448, 512 along z), and 343 = 7^3, the
+# multiples of 64 elements (384, 448, 512 along z), two LDS exchange buffers
+# in the FP32 x groups (280, 320, 384, 448, 512; at 512 with a group's two
+# right-hand sides taking the buffers in turn), and 343 = 7^3, the
 # 16384-monomer workload's padded size (seven-wave x groups of 392
 # butterflies, z inverse groups of five lines, y groups on 448 threads)
 TUNED_CASES = [(P, ax, 3) for P in (280, 320, 336, 343, 384, 448, 512) for ax in (0, 1, 2)]
